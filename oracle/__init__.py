@@ -1,0 +1,287 @@
+"""ctypes binding of the CPU FP64 oracle (oracle/afd_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product path
+(astrofd_amd) never imports this package.
+
+Arrays follow the reference layout: float64, C-order (nv, nx, ny, nz) with ghost
+zones included, where nx, ny, nz are the full dimensions.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I64 = ctypes.c_int64
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("physics", ctypes.c_int), ("nv", ctypes.c_int),
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("ng", ctypes.c_int),
+        ("gamma", ctypes.c_double), ("p_max", ctypes.c_double), ("v_max", ctypes.c_double),
+        ("glm_ch", ctypes.c_double), ("glm_cp", ctypes.c_double),
+        ("limiter_id", ctypes.c_int), ("recon_id", ctypes.c_int), ("riemann_id", ctypes.c_int),
+        ("rk_order", ctypes.c_int), ("n_passive", ctypes.c_int), ("passive_offset", ctypes.c_int),
+        ("resistive", ctypes.c_int), ("resistivity", ctypes.c_double),
+    ]
+
+
+class Nozzle(ctypes.Structure):
+    _fields_ = [
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+        ("y0", ctypes.c_double), ("z0", ctypes.c_double),
+        ("ny_loc", ctypes.c_int), ("nz_loc", ctypes.c_int),
+        ("jet_radius", ctypes.c_double), ("shear_thick", ctypes.c_double), ("profile", ctypes.c_int),
+        ("eta_rho", ctypes.c_double), ("rho_amb", ctypes.c_double), ("p_eq", ctypes.c_double),
+        ("gamma_jet", ctypes.c_double), ("vx_amb", ctypes.c_double), ("vy_amb", ctypes.c_double),
+        ("vz_amb", ctypes.c_double), ("perturb", ctypes.c_int),
+        ("turb_vamp", ctypes.c_double), ("turb_pamp", ctypes.c_double),
+        ("n_tracers", ctypes.c_int), ("tracer_offset", ctypes.c_int),
+        ("tracer_nozzle", _D), ("tracer_amb", _D),
+        ("two_temperature", ctypes.c_int), ("thermo_offset", ctypes.c_int),
+        ("te_nozzle", ctypes.c_double), ("ti_nozzle", ctypes.c_double),
+        ("b_init", ctypes.c_int), ("b0", ctypes.c_double), ("b_amb", ctypes.c_double * 3),
+    ]
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc, no fast-math, no FMA contraction)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER(Params)
+        sig = {
+            "orc_srhd_p2c_batch": [P, _I64, _D, _D], "orc_srhd_c2p_batch": [P, _I64, _D, _D],
+            "orc_srhd_sound_speed_batch": [P, _I64, _D, _D],
+            "orc_srhd_flux_batch": [P, _I64, ctypes.c_int, _D, _D],
+            "orc_ppm": [_D, _D], "orc_srhd_riemann": [P, ctypes.c_int, _D, _D, _D],
+            "orc_rmhd_p2c_batch": [P, _I64, _D, _D], "orc_rmhd_c2p_batch": [P, _I64, _D, _D],
+            "orc_rmhd_hydro_guess_batch": [P, _I64, _D, _D], "orc_rmhd_f_of_Z_batch": [P, _I64, _D, _D, _D],
+            "orc_rmhd_fast_speed_batch": [P, _I64, _D, _D],
+            "orc_rmhd_riemann": [P, ctypes.c_int, _D, _D, ctypes.c_int, _D],
+            "orc_rhs": [P, _D, _D, ctypes.c_double, ctypes.c_double, ctypes.c_double],
+            "orc_step": [P, _D, _D] + [ctypes.c_double] * 4,
+            "orc_apply_periodic_yz": [P, _D], "orc_apply_outflow_right_x": [P, _D],
+            "orc_apply_outflow_yz": [P, _D, ctypes.c_int, ctypes.c_int],
+            "orc_apply_two_temperature": [P, _D, ctypes.c_double, ctypes.c_double, ctypes.c_int],
+        }
+        for name, args in sig.items():
+            getattr(L, name).argtypes = args
+            getattr(L, name).restype = None
+        L.orc_limiter.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.orc_limiter.restype = ctypes.c_double
+        L.orc_weno5_left.argtypes = [_D]; L.orc_weno5_left.restype = ctypes.c_double
+        L.orc_weno5_right.argtypes = [_D]; L.orc_weno5_right.restype = ctypes.c_double
+        L.orc_max_char_speed.argtypes = [P, _D]; L.orc_max_char_speed.restype = ctypes.c_double
+        L.orc_nozzle_weight.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.orc_nozzle_weight.restype = ctypes.c_double
+        L.orc_apply_nozzle_left_x.argtypes = [P, ctypes.POINTER(Nozzle), _D, _D, _I64]
+        L.orc_apply_nozzle_left_x.restype = _I64
+        L.orc_num_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_D)
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+LIMITERS = {"mc": 0, "minmod": 1, "vanleer": 2}
+RECONS = {"muscl": 0, "ppm": 1, "weno": 2}
+
+
+def riemann_id(physics, name):
+    name = str(name).lower()
+    if physics == "rmhd":
+        return 2 if name == "hlld_full" else (1 if name == "hlld" else 0)
+    return 1 if name == "hllc" else 0
+
+
+def make_params(physics="hydro", shape=(1, 1, 1), nv=None, ng=2, gamma=None, p_max=1.0, v_max=0.999,
+                glm_ch=1.0, glm_cp=0.1, limiter="mc", recon="muscl", riemann="hlle", rk_order=2,
+                n_passive=0, passive_offset=None, resistive=False, resistivity=0.0):
+    """Parameters as the reference's configure() leaves them (core/srhd_core.py:18-41,
+    core/rmhd_core.py:21-50).  SRHD always computes with Gamma = 5/3 (Q5)."""
+    rm = physics == "rmhd"
+    nb = 9 if rm else 5
+    if not rm or gamma is None:
+        gamma = 5.0 / 3.0
+    return Params(physics=1 if rm else 0, nv=nv if nv is not None else nb + n_passive,
+                  nx=shape[0], ny=shape[1], nz=shape[2], ng=ng, gamma=gamma, p_max=p_max, v_max=v_max,
+                  glm_ch=glm_ch, glm_cp=glm_cp, limiter_id=LIMITERS.get(limiter, 0), recon_id=RECONS.get(recon, 0),
+                  riemann_id=riemann_id(physics, riemann), rk_order=rk_order, n_passive=n_passive,
+                  passive_offset=passive_offset if passive_offset is not None else nb,
+                  resistive=int(bool(resistive)), resistivity=resistivity)
+
+
+def with_shape(P, pr):
+    Q = Params.from_buffer_copy(P)
+    Q.nv, Q.nx, Q.ny, Q.nz = pr.shape
+    return Q
+
+
+# ---- pointwise ---------------------------------------------------------------
+def srhd_p2c(P, prim):
+    prim = _c(prim); out = np.empty_like(prim)
+    lib().orc_srhd_p2c_batch(ctypes.byref(P), prim.shape[0], _p(prim), _p(out)); return out
+
+
+def srhd_c2p(P, cons):
+    cons = _c(cons); out = np.empty_like(cons)
+    lib().orc_srhd_c2p_batch(ctypes.byref(P), cons.shape[0], _p(cons), _p(out)); return out
+
+
+def srhd_sound_speed(P, prim):
+    prim = _c(prim); out = np.empty(prim.shape[0])
+    lib().orc_srhd_sound_speed_batch(ctypes.byref(P), prim.shape[0], _p(prim), _p(out)); return out
+
+
+def srhd_flux(P, prim, d):
+    prim = _c(prim); out = np.empty_like(prim)
+    lib().orc_srhd_flux_batch(ctypes.byref(P), prim.shape[0], d, _p(prim), _p(out)); return out
+
+
+def srhd_riemann(P, d, qL, qR):
+    qL = _c(qL); qR = _c(qR); out = np.empty(5)
+    lib().orc_srhd_riemann(ctypes.byref(P), d, _p(qL), _p(qR), _p(out)); return out
+
+
+def limiter(lid, a, b):
+    return lib().orc_limiter(lid, a, b)
+
+
+def ppm(q5):
+    q5 = _c(q5); out = np.empty(2); lib().orc_ppm(_p(q5), _p(out)); return out
+
+
+def weno5_left(q5):
+    return lib().orc_weno5_left(_p(_c(q5)))
+
+
+def weno5_right(q5):
+    return lib().orc_weno5_right(_p(_c(q5)))
+
+
+def rmhd_p2c(P, prim):
+    prim = _c(prim); out = np.empty_like(prim)
+    lib().orc_rmhd_p2c_batch(ctypes.byref(P), prim.shape[0], _p(prim), _p(out)); return out
+
+
+def rmhd_c2p(P, cons):
+    cons = _c(cons); out = np.empty_like(cons)
+    lib().orc_rmhd_c2p_batch(ctypes.byref(P), cons.shape[0], _p(cons), _p(out)); return out
+
+
+def rmhd_hydro_guess(P, cons5):
+    cons5 = _c(cons5); out = np.empty_like(cons5)
+    lib().orc_rmhd_hydro_guess_batch(ctypes.byref(P), cons5.shape[0], _p(cons5), _p(out)); return out
+
+
+def rmhd_f_of_Z(P, Z, cons):
+    Z = _c(Z); cons = _c(cons); out = np.empty(Z.shape[0])
+    lib().orc_rmhd_f_of_Z_batch(ctypes.byref(P), Z.shape[0], _p(Z), _p(cons), _p(out)); return out
+
+
+def rmhd_fast_speed(P, prim):
+    prim = _c(prim); out = np.empty(prim.shape[0])
+    lib().orc_rmhd_fast_speed_batch(ctypes.byref(P), prim.shape[0], _p(prim), _p(out)); return out
+
+
+def rmhd_riemann(P, d, qL, qR, unit_speeds=False):
+    qL = _c(qL); qR = _c(qR); out = np.empty(9)
+    lib().orc_rmhd_riemann(ctypes.byref(P), d, _p(qL), _p(qR), int(unit_speeds), _p(out)); return out
+
+
+# ---- whole-array operators ---------------------------------------------------
+def rhs(P, pr, dx, dy, dz):
+    pr = _c(pr); Q = with_shape(P, pr)
+    nb = 9 if Q.physics == 1 else 5
+    out = np.zeros((nb + Q.n_passive,) + pr.shape[1:])
+    lib().orc_rhs(ctypes.byref(Q), _p(pr), _p(out), dx, dy, dz); return out
+
+
+def step(P, pr, dx, dy, dz, dt):
+    pr = _c(pr); Q = with_shape(P, pr); out = np.zeros_like(pr)
+    lib().orc_step(ctypes.byref(Q), _p(pr), _p(out), dx, dy, dz, dt); return out
+
+
+def max_char_speed(P, pr):
+    pr = _c(pr); Q = with_shape(P, pr)
+    return lib().orc_max_char_speed(ctypes.byref(Q), _p(pr))
+
+
+def apply_periodic_yz(P, pr):
+    Q = with_shape(P, pr); lib().orc_apply_periodic_yz(ctypes.byref(Q), _p(pr))
+
+
+def apply_outflow_right_x(P, pr):
+    Q = with_shape(P, pr); lib().orc_apply_outflow_right_x(ctypes.byref(Q), _p(pr))
+
+
+def apply_outflow_yz(P, pr, y=True, z=True):
+    Q = with_shape(P, pr); lib().orc_apply_outflow_yz(ctypes.byref(Q), _p(pr), int(y), int(z))
+
+
+PROFILES = {"top_hat": 0, "taper": 1, "smooth": 1, "parabolic": 2, "parabola": 2}
+B_INITS = {"none": 0, "poloidal": 1, "toroidal": 2}
+
+
+def nozzle_weight(r, radius, shear, profile):
+    return lib().orc_nozzle_weight(r, radius, shear, PROFILES.get(profile, 0))
+
+
+def apply_nozzle_left_x(P, pr, dx, dy, dz, ny_loc, nz_loc, y0, z0, cfg, draws=None, b_amb=(0.0, 0.0, 0.0)):
+    """Restated nozzle fill; `draws` are the uniform [0, 1) numbers the reference would take from
+    its numpy Generator, in order.  Returns the number consumed."""
+    Q = with_shape(P, pr)
+    ntr = int(cfg.get("N_TRACERS", 0))
+    tn = _c(list(cfg.get("TRACER_NOZZLE_VALUES", [])) + [0.0] * 8)
+    ta = _c(list(cfg.get("TRACER_AMB_VALUES", [])) + [0.0] * 8)
+    for t in range(ntr):  # core/nozzle.py:78-79 defaults
+        if t >= len(cfg.get("TRACER_NOZZLE_VALUES", [])):
+            tn[t] = 1.0 if t == 0 else 0.0
+    tt = bool(cfg.get("TWO_TEMPERATURE", False))
+    N = Nozzle(dx=dx, dy=dy, dz=dz, y0=y0, z0=z0, ny_loc=ny_loc, nz_loc=nz_loc,
+               jet_radius=cfg["JET_RADIUS"], shear_thick=cfg["SHEAR_THICK"],
+               profile=PROFILES.get(cfg.get("NOZZLE_PROFILE", "top_hat"), 0),
+               eta_rho=cfg["ETA_RHO"], rho_amb=cfg["RHO_AMB"], p_eq=cfg["P_EQ"], gamma_jet=cfg["GAMMA_JET"],
+               vx_amb=cfg["VX_AMB"], vy_amb=cfg["VY_AMB"], vz_amb=cfg["VZ_AMB"],
+               perturb=0 if cfg.get("NOZZLE_PERTURB", "none") == "none" else 1,
+               turb_vamp=cfg.get("TURB_VAMP", 0.0), turb_pamp=cfg.get("TURB_PAMP", 0.0),
+               n_tracers=ntr, tracer_offset=int(cfg.get("TRACER_OFFSET", 5)),
+               tracer_nozzle=_p(tn), tracer_amb=_p(ta),
+               two_temperature=int(tt), thermo_offset=int(cfg.get("THERMO_OFFSET", 0)),
+               te_nozzle=float(cfg.get("TE_NOZZLE", cfg.get("TE_AMB", 0.0)) or 0.0),
+               ti_nozzle=float(cfg.get("TI_NOZZLE", cfg.get("TI_AMB", 0.0)) or 0.0),
+               b_init=B_INITS.get(cfg.get("B_INIT", "none"), 0), b0=float(cfg.get("B0", 0.0)))
+    N.b_amb[0], N.b_amb[1], N.b_amb[2] = b_amb
+    d = _c(draws if draws is not None else np.zeros(0))
+    return lib().orc_apply_nozzle_left_x(ctypes.byref(Q), ctypes.byref(N), _p(pr), _p(d), d.size)
+
+
+def apply_two_temperature(P, pr, dt, tei_tau, thermo_offset):
+    Q = with_shape(P, pr)
+    lib().orc_apply_two_temperature(ctypes.byref(Q), _p(pr), dt, tei_tau, thermo_offset)
+
+
+def num_threads():
+    return lib().orc_num_threads()

@@ -1,0 +1,178 @@
+"""The CPU oracle against the golden vectors written from the reference itself.
+
+The fixtures in tests/golden/ hold inputs and outputs of the reference's own
+functions (core/srhd_core.py, core/rmhd_core.py, core/boundary.py,
+core/nozzle.py, core/source_terms.py) evaluated as pure Python with numpy;
+tests/golden/make_golden.py wrote them.  The oracle restates that arithmetic
+operation by operation, so the comparison is bitwise (max |diff| == 0).
+"""
+import ast
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from tests import oracle_loop
+
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return np.load(os.path.join(G, name + ".npz"))
+
+
+def assert_bitwise(a, b, what=""):
+    a = np.asarray(a); b = np.asarray(b)
+    assert a.shape == b.shape, what
+    diff = np.abs(a - b)
+    same_nan = np.isnan(a) & np.isnan(b)
+    diff[same_nan] = 0.0
+    assert float(diff.max(initial=0.0)) == 0.0, f"{what}: max|diff| = {diff.max()}"
+
+
+# ---------------------------------------------------------------- pointwise
+def test_srhd_pointwise():
+    f = load("srhd_pointwise")
+    P = O.make_params("hydro", p_max=2.0, v_max=0.9999)
+    assert_bitwise(O.srhd_p2c(P, f["prim"]), f["cons"], "prim_to_cons")
+    assert_bitwise(O.srhd_c2p(P, f["cons_all"]), f["c2p"], "cons_to_prim")
+    assert_bitwise(O.srhd_sound_speed(P, f["prim"]), f["cs"], "sound_speed")
+    fl = np.concatenate([O.srhd_flux(P, f["prim"], d) for d in range(3)], 1)
+    assert_bitwise(fl, f["flux"], "flux_x/y/z")
+    q = f["q"]
+    for name, lid in (("mc", 0), ("minmod", 1), ("vanleer", 2)):
+        assert_bitwise([O.limiter(lid, a[0], a[1]) for a in q], f["lim_" + name], name)
+    assert_bitwise([O.ppm(a) for a in q], f["ppm"], "ppm_reconstruct")
+    assert_bitwise([O.weno5_left(a) for a in q], f["weno_l"], "weno5_left")
+    assert_bitwise([O.weno5_right(a) for a in q], f["weno_r"], "weno5_right")
+    for rid, key in ((0, "hlle"), (1, "hllc")):
+        P.riemann_id = rid
+        out = [np.concatenate([O.srhd_riemann(P, d, f["prim"][a], f["prim"][b]) for d in range(3)])
+               for a, b in zip(f["iL"], f["iR"])]
+        assert_bitwise(out, f[key], key)
+
+
+def test_rmhd_pointwise():
+    f = load("rmhd_pointwise")
+    P = O.make_params("rmhd", gamma=4.0 / 3.0, p_max=1e4, v_max=0.9999)
+    assert_bitwise(O.rmhd_p2c(P, f["prim"]), f["cons"], "prim_to_cons_rmhd")
+    assert_bitwise(O.rmhd_c2p(P, f["cons"]), f["c2p"], "cons_to_prim_rmhd")
+    assert_bitwise(O.rmhd_c2p(P, f["cons_bad"]), f["c2p_bad"], "cons_to_prim_rmhd off-manifold")
+    assert_bitwise(O.rmhd_hydro_guess(P, f["cons"][:, :5]), f["hydro"], "_cons_to_prim_hydro")
+    assert_bitwise(O.rmhd_f_of_Z(P, f["Z"], f["cons"]), f["fZ"], "rmhd_f_of_Z")
+    assert_bitwise(O.rmhd_fast_speed(P, f["prim"]), f["cf"], "rmhd_fast_speed")
+
+
+@pytest.mark.parametrize("rid,key", [(0, "hlle"), (1, "hlld"), (2, "hlld_full")])
+def test_rmhd_riemann(rid, key):
+    f = load("rmhd_riemann")
+    P = O.make_params("rmhd", gamma=4.0 / 3.0, p_max=2.0, v_max=0.9999, recon="weno")
+    P.riemann_id = rid
+    st = f["states"]
+    out = []
+    for m, (a, b) in enumerate(zip(f["iL"], f["iR"])):
+        row = []
+        for d in range(3):
+            qa = st[a].copy(); qb = st[b].copy()
+            if m % 7 == 0:
+                qb[5] = -qa[5]
+            row.append(O.rmhd_riemann(P, d, qa, qb, unit_speeds=(m % 5 == 1)))
+        out.append(np.concatenate(row))
+    assert_bitwise(out, f[key], key)
+
+
+# ---------------------------------------------------------------- whole arrays
+def test_srhd_rhs_and_max_speed():
+    f = load("srhd_rhs")
+    dx, dy, dz = f["dxyz"]
+    for c, name in enumerate(f["cases"]):
+        recon, lim, riem = str(name).split("/")
+        pr = f[f"pr_{c}"]
+        P = O.make_params("hydro", pr.shape[1:], ng=3, p_max=2.0, v_max=0.9999, limiter=lim, recon=recon,
+                          riemann=riem, n_passive=int(f["npass"][c]), passive_offset=5)
+        assert_bitwise(O.rhs(P, pr, dx, dy, dz), f[f"rhs_{c}"], f"rhs {name}")
+        assert O.max_char_speed(P, pr) == float(f[f"amax_{c}"])
+
+
+def test_rmhd_rhs():
+    f = load("rmhd_rhs")
+    dx, dy, dz = f["dxyz"]
+    for c, name in enumerate(f["cases"]):
+        recon, lim, riem = str(name).split("/")
+        res = bool(f["resistive"][c])
+        pr = f[f"pr_{c}"]
+        P = O.make_params("rmhd", pr.shape[1:], ng=3, gamma=4.0 / 3.0, p_max=2.0, v_max=0.9999, limiter=lim,
+                          recon=recon, riemann=riem, resistive=res, resistivity=1e-3 if res else 0.0)
+        assert_bitwise(O.rhs(P, pr, dx, dy, dz), f[f"rhs_{c}"], f"rhs {name} resistive={res}")
+
+
+def test_srhd_step():
+    f = load("srhd_step")
+    dx, dy, dz = f["dxyz"]; dt = float(f["dt"])
+    for c, name in enumerate(f["cases"]):
+        recon, lim, riem, rk, npass = str(name).split("/")
+        pr = f[f"pr_{c}"]
+        P = O.make_params("hydro", pr.shape[1:], ng=3, p_max=2.0, v_max=0.9999, limiter=lim, recon=recon,
+                          riemann=riem, rk_order=int(rk), n_passive=int(npass), passive_offset=5)
+        assert_bitwise(O.step(P, pr, dx, dy, dz, dt), f[f"out_{c}"], f"step {name}")
+
+
+def test_rmhd_step():
+    f = load("rmhd_step")
+    dx, dy, dz = f["dxyz"]; dt = float(f["dt"])
+    for c, name in enumerate(f["cases"]):
+        recon, lim, riem, rk, npass, res = str(name).split("/")
+        res = res == "True"
+        pr = f[f"pr_{c}"]
+        P = O.make_params("rmhd", pr.shape[1:], ng=3, gamma=4.0 / 3.0, p_max=2.0, v_max=0.9999, limiter=lim,
+                          recon=recon, riemann=riem, rk_order=int(rk), n_passive=int(npass), passive_offset=9,
+                          resistive=res, resistivity=1e-3 if res else 0.0)
+        assert_bitwise(O.step(P, pr, dx, dy, dz, dt), f[f"out_{c}"], f"step {name}")
+
+
+# ---------------------------------------------------------------- boundaries
+def test_bc_nozzle_two_temperature():
+    f = load("bc_nozzle")
+    NX, NY, NZ, ng = [int(x) for x in f["grid"]]
+    dx, dy, dz = 1.0 / NX, 1.0 / NY, 1.0 / NZ
+    for c, name in enumerate(f["cases"]):
+        phys, prof, shear, pert, ntr, tt, binit = str(name).split("/")
+        ntr = int(ntr); tt = tt == "True"
+        nvb = 9 if phys == "rmhd" else 5
+        cfg = dict(NG=ng, JET_RADIUS=0.3, SHEAR_THICK=float(shear), NOZZLE_PROFILE=prof, GAMMA_JET=7.0888121,
+                   ETA_RHO=0.01, P_EQ=0.01, RHO_AMB=1.0, VX_AMB=0.01, VY_AMB=-0.02, VZ_AMB=0.03,
+                   NOZZLE_PERTURB=pert, TURB_VAMP=0.02, TURB_PAMP=0.1, N_TRACERS=ntr, TRACER_OFFSET=nvb,
+                   TRACER_NOZZLE_VALUES=[1.0, 0.5][:ntr], TRACER_AMB_VALUES=[0.0, 0.25][:ntr],
+                   TWO_TEMPERATURE=tt, THERMO_OFFSET=nvb + ntr, TE_NOZZLE=0.7, TI_NOZZLE=0.9, PHYSICS=phys,
+                   B_INIT=binit, B0=0.5)
+        pr = f[f"before_{c}"].copy()
+        P = O.make_params(phys, pr.shape[1:], nv=pr.shape[0], ng=ng)
+        O.apply_periodic_yz(P, pr)
+        O.apply_outflow_right_x(P, pr)
+        draws = np.random.default_rng(1234).random(20000)
+        n1 = O.apply_nozzle_left_x(P, pr, dx, dy, dz, NY, NZ, 0.45, 0.55, cfg, draws)
+        n2 = O.apply_nozzle_left_x(P, pr, dx, dy, dz, NY, NZ, 0.45, 0.55, cfg, draws[n1:])
+        assert n1 == n2 and (n1 > 0) == (pert != "none")
+        assert_bitwise(pr, f[f"nozzle_{c}"], f"bcs {name}")
+        if tt:
+            O.apply_two_temperature(P, pr, 0.01, 0.1, nvb + ntr)
+        assert_bitwise(pr, f[f"twotemp_{c}"], f"two-temperature {name}")
+    for prof in ("top_hat", "taper", "parabolic"):
+        for sh in (0.0, 0.02):
+            w = [O.nozzle_weight(x, 0.3, sh, prof) for x in f["r"]]
+            assert_bitwise(w, f[f"w_{prof}_{sh}"], f"nozzle_weight {prof} {sh}")
+
+
+# ---------------------------------------------------------------- driver loops
+@pytest.mark.parametrize("name", ["driver_srhd_10", "driver_rmhd_8", "driver_srhd_32"])
+def test_driver_loop(name):
+    path = os.path.join(G, name + ".npz")
+    if not os.path.exists(path):
+        pytest.skip(f"{name}.npz not generated")
+    f = np.load(path)
+    s = ast.literal_eval(str(f["settings"]))
+    b_amb = (0.0, 0.0, 0.0)
+    out, dts = oracle_loop.run(s, f["pr0"].copy(), len(f["dts"]), b_amb=b_amb)
+    assert_bitwise(dts, f["dts"], "dt sequence")
+    assert_bitwise(out, f["out"], name)
