@@ -1,0 +1,773 @@
+// afd_capi.hip -- the extern "C" entry points of libafd.so (declared in include/afd.h).
+//
+// A context owns the device-resident state of one domain: three primitive buffers
+// (step input and two stage outputs), the conserved states U0 and U_s, the face-flux
+// arrays of the three directions, the nozzle profile and the reduction scratch.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "afd.h"
+#include "afd_launch.hpp"
+#include "afd_mlp.hpp"
+
+using namespace afd;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) return fail(AFD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct afd_ctx {
+    afd_params p{};
+    Grid g{};
+    int nb = 5;
+    bool carried = false, passive_flux = false;
+    size_t fsize = 8;
+    hipStream_t stream = nullptr;
+    double *P[3] = {nullptr, nullptr, nullptr};
+    int cur = 0;
+    double *U0 = nullptr, *Uc = nullptr;
+    void *Fx = nullptr, *Fxc = nullptr, *Fy = nullptr, *Fz = nullptr, *zs = nullptr;
+    double *aux[3] = {nullptr, nullptr, nullptr};
+    double *rhs_pass = nullptr, *rhs_out = nullptr;
+    // nozzle
+    bool noz_set = false;
+    afd_nozzle noz{};
+    double *noz_tmpl = nullptr, *noz_draws = nullptr;
+    int *noz_idx = nullptr;
+    int noz_nact = 0;
+    unsigned noz_mask = 0;
+    // reductions
+    int red_blocks = 1024;
+    double *red_partial = nullptr, *red_out = nullptr;
+    unsigned long long *cnt_partial = nullptr, *cnt_out = nullptr;
+    // instrumentation
+    bool stats_on = false, timing_on = false;
+    unsigned long long *stats_dev = nullptr;
+    afd_stats acc{};
+    struct Mark {
+        int cat;
+        hipEvent_t a, b;
+    };
+    std::vector<Mark> marks;
+    std::vector<hipEvent_t> ev_pool;
+    // learned guess
+    MlpDevice mlp{};
+    std::vector<void *> allocs;
+};
+
+static int dalloc(afd_ctx *c, void **ptr, size_t bytes) {
+    if (bytes == 0) {
+        *ptr = nullptr;
+        return AFD_OK;
+    }
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return fail(AFD_ERR_NOMEM, "hipMalloc(" + std::to_string(bytes) + " B): " + hipGetErrorString(e));
+    }
+    c->allocs.push_back(*ptr);
+    return AFD_OK;
+}
+
+#define ALLOC(ctx, ptr, bytes)                                         \
+    do {                                                               \
+        int rc_ = dalloc((ctx), (void **)&(ptr), (bytes));             \
+        if (rc_ != AFD_OK) return rc_;                                 \
+    } while (0)
+
+static hipEvent_t ev_get(afd_ctx *c) {
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// categories: 0,1,2 face x,y,z; 3 update; 4 other
+struct Timed {
+    afd_ctx *c;
+    int cat;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timed(afd_ctx *ctx, int category) : c(ctx), cat(category) {
+        if (c->timing_on) {
+            a = ev_get(c);
+            b = ev_get(c);
+            if (a && b) (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~Timed() {
+        if (c->timing_on && a && b) {
+            (void)hipEventRecord(b, c->stream);
+            c->marks.push_back({cat, a, b});
+        }
+    }
+};
+
+static int resolve_marks(afd_ctx *c) {
+    if (c->marks.empty()) return AFD_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto &m : c->marks) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, m.a, m.b));
+        if (m.cat < 3) {
+            c->acc.ms_face[m.cat] += ms;
+            c->acc.launches_face[m.cat] += 1;
+        } else if (m.cat == 3) {
+            c->acc.ms_update += ms;
+            c->acc.launches_update += 1;
+        } else {
+            c->acc.ms_other += ms;
+        }
+        c->ev_pool.push_back(m.a);
+        c->ev_pool.push_back(m.b);
+    }
+    c->marks.clear();
+    return AFD_OK;
+}
+
+static Phys<double> make_phys(const afd_params &p) {
+    Phys<double> ph;
+    ph.gamma = p.gamma;
+    ph.gfac = p.gamma / (p.gamma - 1.0);
+    ph.gm1_over_g_inv = p.gamma - 1.0;
+    ph.p_max = p.p_max;
+    ph.v_max = p.v_max;
+    ph.vmax2 = p.v_max * p.v_max;
+    ph.glm_ch2 = p.glm_ch * p.glm_ch;
+    return ph;
+}
+
+static Phys<float> to_float(const Phys<double> &d) {
+    Phys<float> f;
+    f.gamma = (float)d.gamma;
+    f.gfac = (float)d.gfac;
+    f.gm1_over_g_inv = (float)d.gm1_over_g_inv;
+    f.p_max = (float)d.p_max;
+    f.v_max = (float)d.v_max;
+    f.vmax2 = (float)d.vmax2;
+    f.glm_ch2 = (float)d.glm_ch2;
+    return f;
+}
+
+extern "C" const char *afd_last_error(void) { return g_err.c_str(); }
+extern "C" const char *afd_version(void) { return "astrofd_amd 0.1 (gfx950)"; }
+
+extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
+    if (!params || !out) return fail(AFD_ERR_INVALID, "afd_create: null argument");
+    const afd_params &p = *params;
+    if (p.physics != AFD_PHYS_HYDRO && p.physics != AFD_PHYS_RMHD) return fail(AFD_ERR_INVALID, "physics must be 0 or 1");
+    int nb = p.physics == AFD_PHYS_RMHD ? 9 : 5;
+    if (p.ng < 2) return fail(AFD_ERR_INVALID, "NG must be >= 2");
+    if (p.nx < 2 || p.ny < 2 || p.nz < 2) return fail(AFD_ERR_INVALID, "grid too small");
+    if (p.n_passive < 0 || p.nv != nb + p.n_passive) return fail(AFD_ERR_INVALID, "nv must equal 5|9 + n_passive");
+    if (p.n_passive > 0 && p.passive_offset != nb) return fail(AFD_ERR_INVALID, "passive_offset must equal the base variable count");
+    if (p.nv > 32) return fail(AFD_ERR_INVALID, "nv > 32");
+    if (p.recon_id < 0 || p.recon_id > 2) return fail(AFD_ERR_INVALID, "recon_id");
+    if (p.limiter_id < 0 || p.limiter_id > 2) return fail(AFD_ERR_INVALID, "limiter_id");
+    if (p.riemann_id < 0 || p.riemann_id > (nb == 9 ? 2 : 1)) return fail(AFD_ERR_INVALID, "riemann_id");
+    if (p.rk_order != 2 && p.rk_order != 3) return fail(AFD_ERR_INVALID, "rk_order must be 2 or 3");
+    if (p.precision != AFD_PREC_FP64 && p.precision != AFD_PREC_MIXED) return fail(AFD_ERR_INVALID, "precision");
+    if (p.c2p_guess != AFD_C2P_PLAIN && p.c2p_guess != AFD_C2P_LEARNED) return fail(AFD_ERR_INVALID, "c2p_guess");
+    if (p.c2p_guess == AFD_C2P_LEARNED && nb != 9) return fail(AFD_ERR_INVALID, "the learned guess is RMHD only");
+    if (p.two_temperature && (p.thermo_offset < nb || p.thermo_offset + 1 >= p.nv))
+        return fail(AFD_ERR_INVALID, "thermo_offset outside the passive block");
+    if (!(p.dx > 0 && p.dy > 0 && p.dz > 0)) return fail(AFD_ERR_INVALID, "dx, dy, dz must be > 0");
+    if (p.n_resist < 0 || p.n_resist > 2) return fail(AFD_ERR_INVALID, "n_resist must be 0, 1 or 2");
+
+    HIPCHK(hipSetDevice(p.device));
+    afd_ctx *c = new afd_ctx();
+    c->p = p;
+    c->nb = nb;
+    Grid &g = c->g;
+    g.nv = p.nv;
+    g.nx = p.nx + 2 * p.ng;
+    g.ny = p.ny + 2 * p.ng;
+    g.nz = p.nz + 2 * p.ng;
+    g.sy = g.nz;
+    g.sx = (long)g.ny * g.nz;
+    g.ncell = (long)g.nx * g.ny * g.nz;
+    const int full[3] = {g.nx, g.ny, g.nz};
+    for (int a = 0; a < 3; ++a) {
+        // core/srhd_core.py:477-479 [2, n-3); core/rmhd_core.py:751-753 [2, n-2); PPM/WENO [3, n-3)
+        if (p.recon_id == AFD_RECON_MUSCL) {
+            g.r0[a] = 2;
+            g.r1[a] = nb == 9 ? full[a] - 2 : full[a] - 3;
+        } else {
+            g.r0[a] = 3;
+            g.r1[a] = full[a] - 3;
+        }
+        if (g.r1[a] <= g.r0[a]) {
+            delete c;
+            return fail(AFD_ERR_INVALID, "grid too small for the reconstruction stencil");
+        }
+    }
+    c->carried = nb == 9 && p.recon_id != AFD_RECON_MUSCL;
+    c->passive_flux = p.n_passive > 0 && (nb == 9 ? p.recon_id == AFD_RECON_MUSCL : p.recon_id != AFD_RECON_WENO);
+    c->fsize = p.precision == AFD_PREC_MIXED ? 4 : 8;
+    *out = nullptr;
+    int rc;
+#define TRY(x)                       \
+    do {                             \
+        if ((rc = (x)) != AFD_OK) {  \
+            afd_destroy(c);          \
+            return rc;               \
+        }                            \
+    } while (0)
+    size_t N = (size_t)g.ncell;
+    for (int b = 0; b < 3; ++b) TRY(dalloc(c, (void **)&c->P[b], N * p.nv * 8));
+    TRY(dalloc(c, (void **)&c->U0, N * nb * 8));
+    TRY(dalloc(c, (void **)&c->Uc, N * nb * 8));
+    TRY(dalloc(c, &c->Fx, N * nb * c->fsize));
+    TRY(dalloc(c, &c->Fy, N * nb * c->fsize));
+    TRY(dalloc(c, &c->Fz, N * nb * c->fsize));
+    if (c->carried) {
+        TRY(dalloc(c, &c->Fxc, N * nb * c->fsize));
+        TRY(dalloc(c, &c->zs, N * 2 * c->fsize));
+    }
+    if (c->passive_flux) {
+        for (int d = 0; d < 3; ++d) TRY(dalloc(c, (void **)&c->aux[d], N * 4 * 8));
+        TRY(dalloc(c, (void **)&c->rhs_pass, N * p.n_passive * 8));
+    }
+    TRY(dalloc(c, (void **)&c->red_partial, c->red_blocks * 8));
+    TRY(dalloc(c, (void **)&c->red_out, 8));
+    TRY(dalloc(c, (void **)&c->cnt_partial, c->red_blocks * 8));
+    TRY(dalloc(c, (void **)&c->cnt_out, 8));
+    TRY(dalloc(c, (void **)&c->stats_dev, 8 * 8));
+    if (nb == 9) TRY(mlp_alloc(c->mlp, [c](void **ptr, size_t bytes) { return dalloc(c, ptr, bytes); }));
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        afd_destroy(c);
+        return fail(AFD_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    e = hipMemsetAsync(c->stats_dev, 0, 8 * 8, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        afd_destroy(c);
+        return fail(AFD_ERR_HIP, std::string("init: ") + hipGetErrorString(e));
+    }
+#undef TRY
+    *out = c;
+    return AFD_OK;
+}
+
+extern "C" int afd_destroy(afd_ctx *c) {
+    if (!c) return AFD_OK;
+    (void)hipSetDevice(c->p.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &m : c->marks) {
+        (void)hipEventDestroy(m.a);
+        (void)hipEventDestroy(m.b);
+    }
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    for (void *ptr : c->allocs) (void)hipFree(ptr);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return AFD_OK;
+}
+
+static int check_n(afd_ctx *c, int64_t n) {
+    if (n != (int64_t)c->g.ncell * c->p.nv)
+        return fail(AFD_ERR_INVALID, "array size " + std::to_string(n) + " != nv * cells " +
+                                          std::to_string((int64_t)c->g.ncell * c->p.nv));
+    return AFD_OK;
+}
+
+extern "C" int afd_set_prim(afd_ctx *c, const double *host, int64_t n) {
+    if (!c || !host) return fail(AFD_ERR_INVALID, "afd_set_prim: null argument");
+    if (int rc = check_n(c, n)) return rc;
+    HIPCHK(hipSetDevice(c->p.device));
+    HIPCHK(hipMemcpyAsync(c->P[c->cur], host, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AFD_OK;
+}
+
+extern "C" int afd_get_prim(afd_ctx *c, double *host, int64_t n) {
+    if (!c || !host) return fail(AFD_ERR_INVALID, "afd_get_prim: null argument");
+    if (int rc = check_n(c, n)) return rc;
+    HIPCHK(hipSetDevice(c->p.device));
+    HIPCHK(hipMemcpyAsync(host, c->P[c->cur], (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AFD_OK;
+}
+
+// ------------------------------------------------------------------ nozzle
+static double (*volatile libm_pow)(double, double) = pow;
+
+// core/nozzle.py:5-25
+static double nozzle_weight(double r, double radius, double shear_thick, int profile) {
+#pragma clang fp contract(off)
+    if (radius <= 0.0) return 0.0;
+    if (profile == 1) {
+        if (r >= radius) return 0.0;
+        return 0.5 * (1.0 + cos(M_PI * r / radius));
+    }
+    if (profile == 2) {
+        if (r >= radius) return 0.0;
+        double x = r / radius;
+        double w = 1.0 - x * x;
+        return (w > 0.0) ? w : 0.0;
+    }
+    if (shear_thick > 0.0) return 0.5 * (1.0 - tanh((r - radius) / shear_thick));
+    return (r <= radius) ? 1.0 : 0.0;
+}
+
+extern "C" int afd_set_nozzle(afd_ctx *c, const afd_nozzle *nz) {
+#pragma clang fp contract(off)
+    if (!c || !nz) return fail(AFD_ERR_INVALID, "afd_set_nozzle: null argument");
+    const afd_params &p = c->p;
+    if (nz->n_tracers < 0 || nz->n_tracers > 8) return fail(AFD_ERR_INVALID, "n_tracers must be in [0, 8]");
+    if (nz->n_tracers > 0 && (nz->tracer_offset < 0 || nz->tracer_offset + nz->n_tracers > p.nv))
+        return fail(AFD_ERR_INVALID, "tracer block outside nv");
+    const int NY = p.ny, NZ = p.nz, nv = p.nv;
+    const long plane = (long)NY * NZ;
+    std::vector<double> tmpl((size_t)nv * plane, 0.0);
+    std::vector<int> idx((size_t)plane, -1);
+    unsigned mask = 0x1f;  // rho, vx, vy, vz, p
+    for (int t = 0; t < nz->n_tracers; ++t) mask |= 1u << (nz->tracer_offset + t);
+    bool tt = nz->two_temperature && p.nv > nz->thermo_offset + 1;
+    if (tt) mask |= (1u << nz->thermo_offset) | (1u << (nz->thermo_offset + 1));
+    if (c->nb == 9) mask |= 0x1e0u;
+    int nact = 0;
+    double beta = sqrt(1.0 - 1.0 / (nz->gamma_jet * nz->gamma_jet));
+    for (int jj = 0; jj < NY; ++jj) {
+        double y = (jj + 0.5) * p.dy;
+        for (int kk = 0; kk < NZ; ++kk) {
+            double z = (kk + 0.5) * p.dz;
+            double rr = sqrt(libm_pow(y - nz->y0, 2.0) + libm_pow(z - nz->z0, 2.0));
+            double s = nozzle_weight(rr, nz->jet_radius, nz->shear_thick, nz->profile);
+            long o = (long)jj * NZ + kk;
+            tmpl[0 * plane + o] = nz->eta_rho * nz->rho_amb * s + nz->rho_amb * (1.0 - s);
+            tmpl[1 * plane + o] = beta * s + nz->vx_amb * (1.0 - s);
+            tmpl[2 * plane + o] = nz->vy_amb;
+            tmpl[3 * plane + o] = nz->vz_amb;
+            tmpl[4 * plane + o] = nz->p_eq;
+            for (int t = 0; t < nz->n_tracers; ++t) {
+                double tn = nz->tracer_nozzle[t], ta = nz->tracer_amb[t];
+                tmpl[(size_t)(nz->tracer_offset + t) * plane + o] = ta + (tn - ta) * s;
+            }
+            if (tt) {
+                tmpl[(size_t)nz->thermo_offset * plane + o] = nz->te_nozzle;
+                tmpl[(size_t)(nz->thermo_offset + 1) * plane + o] = nz->ti_nozzle;
+            }
+            if (c->nb == 9) {
+                double Bx = 0.0, By = 0.0, Bz = 0.0;
+                if (nz->b_init == 1) Bx = nz->b0 * s;
+                else if (nz->b_init == 2) By = nz->b0 * s;
+                tmpl[5 * plane + o] = Bx + nz->b_amb[0];
+                tmpl[6 * plane + o] = By + nz->b_amb[1];
+                tmpl[7 * plane + o] = Bz + nz->b_amb[2];
+                tmpl[8 * plane + o] = 0.0;
+            }
+            if (s > 0.0) idx[o] = nact++;
+        }
+    }
+    HIPCHK(hipSetDevice(p.device));
+    if (!c->noz_tmpl) {
+        ALLOC(c, c->noz_tmpl, tmpl.size() * 8);
+        ALLOC(c, c->noz_idx, idx.size() * sizeof(int));
+        ALLOC(c, c->noz_draws, (size_t)3 * p.ng * plane * 8 + 8);
+    }
+    HIPCHK(hipMemcpyAsync(c->noz_tmpl, tmpl.data(), tmpl.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->noz_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->noz = *nz;
+    c->noz_nact = nact;
+    c->noz_mask = mask;
+    c->noz_set = true;
+    return AFD_OK;
+}
+
+extern "C" int afd_nozzle_draw_count(afd_ctx *c, int64_t *n) {
+    if (!c || !n) return fail(AFD_ERR_INVALID, "afd_nozzle_draw_count: null argument");
+    *n = (c->noz_set && c->noz.perturb) ? (int64_t)3 * c->p.ng * c->noz_nact : 0;
+    return AFD_OK;
+}
+
+static int fill_yz(afd_ctx *c, int which) {
+    BcArgs a{c->P[c->cur], c->g, c->p.ng, 0};
+    Timed t(c, 4);
+    if (which & 1) {
+        a.bc = c->p.bc_y;
+        hipLaunchKernelGGL(k_fill_y, grid1d((long)c->g.nv * c->g.nx * 2 * c->p.ng * c->g.nz), dim3(BLOCK), 0,
+                           c->stream, a);
+    }
+    if (which & 2) {
+        a.bc = c->p.bc_z;
+        hipLaunchKernelGGL(k_fill_z, grid1d((long)c->g.nv * c->g.nx * c->g.ny * 2 * c->p.ng), dim3(BLOCK), 0,
+                           c->stream, a);
+    }
+    HIPCHK(hipGetLastError());
+    return AFD_OK;
+}
+
+static int outflow_x(afd_ctx *c) {
+    BcArgs a{c->P[c->cur], c->g, c->p.ng, 1};
+    Timed t(c, 4);
+    hipLaunchKernelGGL(k_outflow_x, grid1d((long)c->g.nv * c->p.ng * c->g.ny * c->g.nz), dim3(BLOCK), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    return AFD_OK;
+}
+
+static int nozzle_fill(afd_ctx *c, const double *draws, int64_t ndraws) {
+    if (!c->noz_set) return fail(AFD_ERR_STATE, "nozzle not set (afd_set_nozzle)");
+    int64_t need = 0;
+    afd_nozzle_draw_count(c, &need);
+    if (need > 0) {
+        if (!draws || ndraws < need)
+            return fail(AFD_ERR_INVALID, "perturbed nozzle needs " + std::to_string(need) + " draws");
+        HIPCHK(hipMemcpyAsync(c->noz_draws, draws, (size_t)need * 8, hipMemcpyHostToDevice, c->stream));
+    }
+    NozzleArgs a;
+    a.P = c->P[c->cur];
+    a.tmpl = c->noz_tmpl;
+    a.idx = c->noz_idx;
+    a.draws = c->noz_draws;
+    a.g = c->g;
+    a.ng = c->p.ng;
+    a.ny_int = c->p.ny;
+    a.nz_int = c->p.nz;
+    a.nact = c->noz_nact;
+    a.perturb = need > 0;
+    a.mask = c->noz_mask;
+    a.vamp = c->noz.turb_vamp;
+    a.pamp = c->noz.turb_pamp;
+    Timed t(c, 4);
+    hipLaunchKernelGGL(k_nozzle, grid1d((long)c->p.ng * c->p.ny * c->p.nz), dim3(BLOCK), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    if (need > 0) HIPCHK(hipStreamSynchronize(c->stream));  // the host draw buffer may be reused
+    return AFD_OK;
+}
+
+extern "C" int afd_apply_periodic_yz(afd_ctx *c) {
+    if (!c) return fail(AFD_ERR_INVALID, "null context");
+    HIPCHK(hipSetDevice(c->p.device));
+    int by = c->p.bc_y, bz = c->p.bc_z;
+    c->p.bc_y = c->p.bc_z = AFD_BC_PERIODIC;
+    int rc = fill_yz(c, 3);
+    c->p.bc_y = by;
+    c->p.bc_z = bz;
+    return rc;
+}
+
+extern "C" int afd_apply_outflow_right_x(afd_ctx *c) {
+    if (!c) return fail(AFD_ERR_INVALID, "null context");
+    HIPCHK(hipSetDevice(c->p.device));
+    return outflow_x(c);
+}
+
+extern "C" int afd_apply_nozzle_left_x(afd_ctx *c, const double *draws, int64_t ndraws) {
+    if (!c) return fail(AFD_ERR_INVALID, "null context");
+    HIPCHK(hipSetDevice(c->p.device));
+    return nozzle_fill(c, draws, ndraws);
+}
+
+extern "C" int afd_apply_bcs(afd_ctx *c, const double *draws, int64_t ndraws) {
+    if (!c) return fail(AFD_ERR_INVALID, "null context");
+    HIPCHK(hipSetDevice(c->p.device));
+    int rc;
+    if ((rc = fill_yz(c, 3))) return rc;
+    if ((rc = outflow_x(c))) return rc;
+    if (c->noz_set && (rc = nozzle_fill(c, draws, ndraws))) return rc;
+    return AFD_OK;
+}
+
+// ------------------------------------------------------------------ RHS / step
+static FaceLaunch face_launch(afd_ctx *c, const double *P) {
+    FaceLaunch l;
+    l.P = P;
+    l.F = l.Fcar = l.zs = nullptr;
+    l.aux = nullptr;
+    l.g = c->g;
+    l.phd = make_phys(c->p);
+    l.phf = to_float(l.phd);
+    l.p_max = c->p.p_max;
+    l.v_max = c->p.v_max;
+    l.lim = c->p.limiter_id;
+    l.recon = c->p.recon_id;
+    l.riemann = c->p.riemann_id;
+    l.dir = 0;
+    l.mixed = c->p.precision == AFD_PREC_MIXED;
+    return l;
+}
+
+static int run_faces(afd_ctx *c, const double *P) {
+    FaceLaunch l = face_launch(c, P);
+    void *Fd[3] = {c->Fx, c->Fy, c->Fz};
+    const int order_carried[3] = {2, 0, 1}, order_plain[3] = {0, 1, 2};
+    const int *order = c->carried ? order_carried : order_plain;
+    for (int n = 0; n < 3; ++n) {
+        int d = order[n];
+        l.dir = d;
+        l.F = Fd[d];
+        l.Fcar = (c->carried && d == 0) ? c->Fxc : nullptr;
+        l.zs = c->carried ? c->zs : nullptr;
+        l.aux = c->passive_flux ? c->aux[d] : nullptr;
+        Timed t(c, d);
+        hipError_t e = (c->nb == 9) ? launch_face_rmhd(l, c->stream) : launch_face_srhd(l, c->stream);
+        if (e != hipSuccess) return fail(AFD_ERR_HIP, std::string("face kernel: ") + hipGetErrorString(e));
+    }
+    if (c->passive_flux) {
+        PassiveArgs a;
+        a.P = P;
+        for (int d = 0; d < 3; ++d) a.aux[d] = c->aux[d];
+        a.rhs = c->rhs_pass;
+        a.g = c->g;
+        a.physics = c->p.physics;
+        a.recon = c->p.recon_id;
+        a.lim = c->p.limiter_id;
+        a.np = c->p.n_passive;
+        a.po = c->p.passive_offset;
+        a.d[0] = c->p.dx;
+        a.d[1] = c->p.dy;
+        a.d[2] = c->p.dz;
+        Timed t(c, 4);
+        hipLaunchKernelGGL(k_passive_rhs, grid1d(c->g.ncell), dim3(BLOCK), 0, c->stream, a);
+        HIPCHK(hipGetLastError());
+    }
+    return AFD_OK;
+}
+
+static UpdLaunch upd_launch(afd_ctx *c, const double *Pin, const double *P0, double *Pout, int stage, int nst,
+                            double dt) {
+    UpdLaunch l;
+    l.nb = c->nb;
+    l.mixed = c->p.precision == AFD_PREC_MIXED;
+    UpdArgs<double> &a = l.a;
+    a.P0 = P0;
+    a.Pin = Pin;
+    a.Pout = Pout;
+    a.U0 = c->U0;
+    a.Uc = c->Uc;
+    a.Fx = (const double *)c->Fx;
+    a.FxL = (const double *)(c->carried ? c->Fxc : c->Fx);
+    a.Fy = (const double *)c->Fy;
+    a.Fz = (const double *)c->Fz;
+    a.rhs_pass = c->passive_flux ? c->rhs_pass : nullptr;
+    a.stats = c->stats_on ? c->stats_dev : nullptr;
+    a.g = c->g;
+    a.stage = stage;
+    a.nst = nst;
+    a.n_resist = c->nb == 9 ? c->p.n_resist : 0;
+    a.np = c->p.n_passive;
+    a.po = c->p.passive_offset;
+    a.dt = dt;
+    a.dx = c->p.dx;
+    a.dy = c->p.dy;
+    a.dz = c->p.dz;
+    a.glm_cp2 = c->p.glm_cp * c->p.glm_cp;
+    a.resistivity = c->p.resistivity;
+    a.gamma = c->p.gamma;
+    a.gfac = c->p.gamma / (c->p.gamma - 1.0);
+    a.p_max = c->p.p_max;
+    a.v_max = c->p.v_max;
+    a.mlp = c->p.c2p_guess == AFD_C2P_LEARNED ? c->mlp.w : nullptr;
+    return l;
+}
+
+extern "C" int afd_step(afd_ctx *c, double dt) {
+    if (!c) return fail(AFD_ERR_INVALID, "null context");
+    if (!(dt >= 0.0) || !std::isfinite(dt)) return fail(AFD_ERR_INVALID, "dt must be finite and >= 0");
+    if (c->p.c2p_guess == AFD_C2P_LEARNED && !c->mlp.loaded)
+        return fail(AFD_ERR_STATE, "learned guess selected but no weights loaded (afd_mlp_set_weights)");
+    HIPCHK(hipSetDevice(c->p.device));
+    const int nst = c->p.rk_order == 3 ? 3 : 2;
+    const int in = c->cur, a = (in + 1) % 3, b = (in + 2) % 3;
+    // SSPRK2: in -> a -> b ; SSPRK3: in -> a -> b -> a.  P[in] stays the step input.
+    int src = in;
+    for (int s = 0; s < nst; ++s) {
+        int dst = (s % 2 == 0) ? a : b;
+        int rc = run_faces(c, c->P[src]);
+        if (rc) return rc;
+        UpdLaunch l = upd_launch(c, c->P[src], c->P[in], c->P[dst], s, nst, dt);
+        Timed t(c, 3);
+        hipError_t e = launch_update(l, false, c->stream);
+        if (e != hipSuccess) return fail(AFD_ERR_HIP, std::string("update kernel: ") + hipGetErrorString(e));
+        src = dst;
+    }
+    c->cur = src;
+    c->acc.c2p_cells += (uint64_t)nst * c->g.ncell;
+    return AFD_OK;
+}
+
+extern "C" int afd_compute_rhs(afd_ctx *c, double *host, int64_t n) {
+    if (!c || !host) return fail(AFD_ERR_INVALID, "afd_compute_rhs: null argument");
+    int nrhs = c->nb + c->p.n_passive;
+    if (n != (int64_t)nrhs * c->g.ncell) return fail(AFD_ERR_INVALID, "rhs size must be (nb + n_passive) * cells");
+    HIPCHK(hipSetDevice(c->p.device));
+    if (!c->rhs_out) ALLOC(c, c->rhs_out, (size_t)nrhs * c->g.ncell * 8);
+    int rc = run_faces(c, c->P[c->cur]);
+    if (rc) return rc;
+    UpdLaunch l = upd_launch(c, c->P[c->cur], c->P[c->cur], c->rhs_out, 0, 2, 0.0);
+    hipError_t e = launch_update(l, true, c->stream);
+    if (e != hipSuccess) return fail(AFD_ERR_HIP, std::string("rhs kernel: ") + hipGetErrorString(e));
+    HIPCHK(hipMemcpyAsync(host, c->rhs_out, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AFD_OK;
+}
+
+extern "C" int afd_two_temperature(afd_ctx *c, double dt) {
+    if (!c) return fail(AFD_ERR_INVALID, "null context");
+    if (!c->p.two_temperature || c->p.tei_tau <= 0.0) return AFD_OK;  // core/source_terms.py:34-42
+    HIPCHK(hipSetDevice(c->p.device));
+    TwoTempArgs a{c->P[c->cur], c->g, c->p.ng, c->p.thermo_offset, dt, c->p.tei_tau};
+    Timed t(c, 4);
+    hipLaunchKernelGGL(k_two_temperature, grid1d(c->g.ncell), dim3(BLOCK), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    return AFD_OK;
+}
+
+extern "C" int afd_max_char_speed(afd_ctx *c, double *amax) {
+    if (!c || !amax) return fail(AFD_ERR_INVALID, "afd_max_char_speed: null argument");
+    HIPCHK(hipSetDevice(c->p.device));
+    MaxSpeedArgs a{c->P[c->cur], c->red_partial, c->g, c->p.gamma, c->p.gamma / (c->p.gamma - 1.0)};
+    {
+        Timed t(c, 4);
+        hipLaunchKernelGGL(k_max_speed, dim3(c->red_blocks), dim3(BLOCK), 0, c->stream, a);
+        hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(BLOCK), 0, c->stream, (const double *)c->red_partial,
+                           c->red_blocks, SMALL, c->red_out);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(amax, c->red_out, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AFD_OK;
+}
+
+extern "C" int afd_count_nonfinite(afd_ctx *c, int64_t *count) {
+    if (!c || !count) return fail(AFD_ERR_INVALID, "afd_count_nonfinite: null argument");
+    HIPCHK(hipSetDevice(c->p.device));
+    long n = c->g.ncell * c->p.nv;
+    hipLaunchKernelGGL(k_count_nonfinite, dim3(c->red_blocks), dim3(BLOCK), 0, c->stream,
+                       (const double *)c->P[c->cur], n, c->cnt_partial);
+    hipLaunchKernelGGL(k_reduce_sum_u64, dim3(1), dim3(BLOCK), 0, c->stream,
+                       (const unsigned long long *)c->cnt_partial, c->red_blocks, c->cnt_out);
+    HIPCHK(hipGetLastError());
+    unsigned long long v = 0;
+    HIPCHK(hipMemcpyAsync(&v, c->cnt_out, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *count = (int64_t)v;
+    return AFD_OK;
+}
+
+static int batch(afd_ctx *c, int64_t n, const double *in, double *out, int32_t *iters, bool c2p) {
+    if (!c || (n > 0 && (!in || !out))) return fail(AFD_ERR_INVALID, "batch: null argument");
+    if (n < 0) return fail(AFD_ERR_INVALID, "batch: n < 0");
+    if (n == 0) return AFD_OK;
+    HIPCHK(hipSetDevice(c->p.device));
+    size_t bytes = (size_t)n * c->nb * 8;
+    double *din = nullptr, *dout = nullptr;
+    int *dit = nullptr;
+    HIPCHK(hipMalloc(&din, bytes));
+    hipError_t e = hipMalloc(&dout, bytes);
+    if (e == hipSuccess && iters) e = hipMalloc(&dit, (size_t)n * 4);
+    if (e == hipSuccess) e = hipMemcpyAsync(din, in, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        BatchArgs a{din, dout, dit, (long)n, c->p.physics, c->p.gamma, c->p.gamma / (c->p.gamma - 1.0), c->p.p_max,
+                    c->p.v_max};
+        if (c2p) {
+            C2PBatchExtra x{c->p.c2p_guess == AFD_C2P_LEARNED && c->mlp.loaded ? c->mlp.w : nullptr};
+            launch_c2p_batch(a, x, c->stream);
+        } else {
+            hipLaunchKernelGGL(k_p2c_batch, grid1d(n), dim3(BLOCK), 0, c->stream, a);
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && iters) e = hipMemcpyAsync(iters, dit, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    if (dit) (void)hipFree(dit);
+    if (e != hipSuccess) return fail(AFD_ERR_HIP, std::string("batch: ") + hipGetErrorString(e));
+    return AFD_OK;
+}
+
+extern "C" int afd_p2c_batch(afd_ctx *c, int64_t n, const double *prim, double *cons) {
+    return batch(c, n, prim, cons, nullptr, false);
+}
+extern "C" int afd_c2p_batch(afd_ctx *c, int64_t n, const double *cons, double *prim, int32_t *iters) {
+    return batch(c, n, cons, prim, iters, true);
+}
+
+// ------------------------------------------------------------------ instrumentation
+extern "C" int afd_set_instrumentation(afd_ctx *c, int32_t stats, int32_t timing) {
+    if (!c) return fail(AFD_ERR_INVALID, "null context");
+    c->stats_on = stats != 0;
+    c->timing_on = timing != 0;
+    return AFD_OK;
+}
+
+extern "C" int afd_get_stats(afd_ctx *c, afd_stats *out) {
+    if (!c || !out) return fail(AFD_ERR_INVALID, "afd_get_stats: null argument");
+    HIPCHK(hipSetDevice(c->p.device));
+    if (int rc = resolve_marks(c)) return rc;
+    unsigned long long v[8] = {0};
+    HIPCHK(hipMemcpyAsync(v, c->stats_dev, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *out = c->acc;
+    out->hydro_iters = v[0];
+    out->newton_evals = v[1];
+    out->hydro_fallbacks = v[2];
+    out->bisections = v[3];
+    out->invalid = v[4];
+    out->learned_ok = v[5];
+    return AFD_OK;
+}
+
+extern "C" int afd_reset_stats(afd_ctx *c) {
+    if (!c) return fail(AFD_ERR_INVALID, "null context");
+    HIPCHK(hipSetDevice(c->p.device));
+    if (int rc = resolve_marks(c)) return rc;
+    HIPCHK(hipMemsetAsync(c->stats_dev, 0, 8 * 8, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->acc = afd_stats{};
+    return AFD_OK;
+}
+
+extern "C" int afd_synchronize(afd_ctx *c) {
+    if (!c) return fail(AFD_ERR_INVALID, "null context");
+    HIPCHK(hipSetDevice(c->p.device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AFD_OK;
+}
+
+// ------------------------------------------------------------------ learned guess weights
+extern "C" int afd_mlp_param_count(int32_t *count) {
+    if (!count) return fail(AFD_ERR_INVALID, "null argument");
+    *count = MLP_NPARAM;
+    return AFD_OK;
+}
+
+extern "C" int afd_mlp_set_weights(afd_ctx *c, const float *w, int32_t count) {
+    if (!c || !w) return fail(AFD_ERR_INVALID, "afd_mlp_set_weights: null argument");
+    if (c->nb != 9) return fail(AFD_ERR_STATE, "the learned guess is RMHD only");
+    if (count != MLP_NPARAM) return fail(AFD_ERR_INVALID, "expected " + std::to_string(MLP_NPARAM) + " weights");
+    HIPCHK(hipSetDevice(c->p.device));
+    HIPCHK(mlp_upload(c->mlp, w, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->mlp.loaded = true;
+    return AFD_OK;
+}
+
+extern "C" int afd_mlp_get_weights(afd_ctx *c, float *w, int32_t count) {
+    if (!c || !w) return fail(AFD_ERR_INVALID, "afd_mlp_get_weights: null argument");
+    if (c->nb != 9 || !c->mlp.loaded) return fail(AFD_ERR_STATE, "no weights loaded");
+    if (count != MLP_NPARAM) return fail(AFD_ERR_INVALID, "expected " + std::to_string(MLP_NPARAM) + " weights");
+    HIPCHK(hipSetDevice(c->p.device));
+    HIPCHK(hipMemcpyAsync(w, c->mlp.master, MLP_NPARAM * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AFD_OK;
+}

@@ -1,0 +1,290 @@
+// afd_common.hip -- stage update / rhs kernels and the RMHD face dispatcher.
+#include "afd_launch.hpp"
+
+namespace afd {
+
+// ------------------------------------------------------------------ non-template kernels
+__global__ __launch_bounds__(BLOCK) void k_passive_rhs(PassiveArgs a) {
+#pragma clang fp contract(off)
+    const Grid &g = a.g;
+    long c = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= g.ncell) return;
+    int k = (int)(c % g.nz), j = (int)((c / g.nz) % g.ny), i = (int)(c / ((long)g.ny * g.nz));
+    bool in = i >= g.r0[0] && i < g.r1[0] && j >= g.r0[1] && j < g.r1[1] && k >= g.r0[2] && k < g.r1[2];
+    for (int t = 0; t < a.np; ++t) {
+        double r = 0.0;
+        if (in) {
+            for (int dir = 0; dir < 3; ++dir) {
+                // which variants the reference evaluates per direction, and which left flux survives
+                int nvar;           // right-face variants subtracted, in order WENO, PPM, MUSCL
+                int vars[3];
+                int left;           // variant of the surviving left flux
+                if (a.physics == 1) {            // RMHD MUSCL: all three, every direction
+                    nvar = 3; vars[0] = 0; vars[1] = 1; vars[2] = 2; left = 2;
+                } else if (a.recon == 0) {       // SRHD MUSCL: x all three, y none, z MUSCL
+                    if (dir == 1) continue;
+                    if (dir == 0) { nvar = 3; vars[0] = 0; vars[1] = 1; vars[2] = 2; left = 2; }
+                    else { nvar = 1; vars[0] = 2; left = 2; }
+                } else {                         // SRHD PPM: x none, y all three, z WENO and PPM
+                    if (dir == 0) continue;
+                    if (dir == 1) { nvar = 3; vars[0] = 0; vars[1] = 1; vars[2] = 2; left = 2; }
+                    else { nvar = 2; vars[0] = 0; vars[1] = 1; left = 1; }
+                }
+                const double *ax = a.aux[dir];
+                long cp = c + (dir == 0 ? g.sx : (dir == 1 ? g.sy : 1L));
+                int ip = i + (dir == 0), jp = j + (dir == 1), kp = k + (dir == 2);
+                double sLa = ax[c], sRa = ax[g.ncell + c], vLa = ax[2 * g.ncell + c], vRa = ax[3 * g.ncell + c];
+                double sLb = ax[cp], sRb = ax[g.ncell + cp], vLb = ax[2 * g.ncell + cp], vRb = ax[3 * g.ncell + cp];
+                if (a.physics == 1) {  // right faces keep the left face's speeds (core/rmhd_core.py:915)
+                    sLb = sLa;
+                    sRb = sRa;
+                }
+                double FLt = passive_flux(a, left, a.po + t, dir, i, j, k, vLa, vRa, sLa, sRa);
+                for (int m = 0; m < nvar; ++m) {
+                    double FRt = passive_flux(a, vars[m], a.po + t, dir, ip, jp, kp, vLb, vRb, sLb, sRb);
+                    r -= (FRt - FLt) / a.d[dir];
+                }
+            }
+        }
+        a.rhs[t * g.ncell + c] = r;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_c2p_batch(BatchArgs a) {
+    long m = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (m >= a.n) return;
+    C2PStats st = {0, 0, 0};
+    if (a.physics == 1) {
+        double U[9], q[9];
+        for (int v = 0; v < 9; ++v) U[v] = a.in[9 * m + v];
+        RmhdC2PParams cp = {a.gamma, a.gfac, a.p_max, a.v_max};
+        rmhd_c2p(cp, U, q, st);
+        for (int v = 0; v < 9; ++v) a.out[9 * m + v] = q[v];
+    } else {
+        double U[5], q[5];
+        for (int v = 0; v < 5; ++v) U[v] = a.in[5 * m + v];
+        hydro_c2p(a.gamma, a.gfac, a.p_max, a.v_max, U, q, st);
+        for (int v = 0; v < 5; ++v) a.out[5 * m + v] = q[v];
+    }
+    if (a.iters) a.iters[m] = st.hydro_iters + st.newton_iters;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_p2c_batch(BatchArgs a) {
+    long m = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (m >= a.n) return;
+    if (a.physics == 1) {
+        double q[9], U[9];
+        for (int v = 0; v < 9; ++v) q[v] = a.in[9 * m + v];
+        rmhd_p2c_cell(a.gfac, a.p_max, a.v_max, q, U);
+        for (int v = 0; v < 9; ++v) a.out[9 * m + v] = U[v];
+    } else {
+        double q[5], U[5];
+        for (int v = 0; v < 5; ++v) q[v] = a.in[5 * m + v];
+        srhd_p2c_cell(a.gfac, q, U);
+        for (int v = 0; v < 5; ++v) a.out[5 * m + v] = U[v];
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_fill_y(BcArgs a) {
+#pragma clang fp contract(off)
+    const Grid &g = a.g;
+    long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    long tot = (long)g.nv * g.nx * 2 * a.ng * g.nz;
+    if (t >= tot) return;
+    int k = (int)(t % g.nz);
+    long r = t / g.nz;
+    int gg = (int)(r % (2 * a.ng));
+    r /= 2 * a.ng;
+    int i = (int)(r % g.nx);
+    int v = (int)(r / g.nx);
+    int jd, js;
+    if (gg < a.ng) {
+        jd = gg;
+        js = a.bc ? a.ng : g.ny - 2 * a.ng + gg;
+    } else {
+        jd = g.ny - a.ng + (gg - a.ng);
+        js = a.bc ? g.ny - a.ng - 1 : a.ng + (gg - a.ng);
+    }
+    double *b = a.P + (long)v * g.ncell;
+    b[cidx(g, i, jd, k)] = b[cidx(g, i, js, k)];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_fill_z(BcArgs a) {
+    const Grid &g = a.g;
+    long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    long tot = (long)g.nv * g.nx * g.ny * 2 * a.ng;
+    if (t >= tot) return;
+    int gg = (int)(t % (2 * a.ng));
+    long r = t / (2 * a.ng);
+    int j = (int)(r % g.ny);
+    r /= g.ny;
+    int i = (int)(r % g.nx);
+    int v = (int)(r / g.nx);
+    int kd, ks;
+    if (gg < a.ng) {
+        kd = gg;
+        ks = a.bc ? a.ng : g.nz - 2 * a.ng + gg;
+    } else {
+        kd = g.nz - a.ng + (gg - a.ng);
+        ks = a.bc ? g.nz - a.ng - 1 : a.ng + (gg - a.ng);
+    }
+    double *b = a.P + (long)v * g.ncell;
+    b[cidx(g, i, j, kd)] = b[cidx(g, i, j, ks)];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_outflow_x(BcArgs a) {
+    const Grid &g = a.g;
+    long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    long plane = (long)g.ny * g.nz;
+    long tot = (long)g.nv * a.ng * plane;
+    if (t >= tot) return;
+    long jk = t % plane;
+    long r = t / plane;
+    int gi = (int)(r % a.ng);
+    int v = (int)(r / a.ng);
+    double *b = a.P + (long)v * g.ncell;
+    b[(long)(g.nx - a.ng + gi) * plane + jk] = b[(long)(g.nx - a.ng - 1) * plane + jk];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_nozzle(NozzleArgs a) {
+#pragma clang fp contract(off)
+    const Grid &g = a.g;
+    long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    long plane = (long)a.ny_int * a.nz_int;
+    if (t >= (long)a.ng * plane) return;
+    long jk = t % plane;
+    int gi = (int)(t / plane);
+    int jj = (int)(jk / a.nz_int), kk = (int)(jk % a.nz_int);
+    long c = cidx(g, gi, a.ng + jj, a.ng + kk);
+    for (int v = 0; v < g.nv && v < 32; ++v) {
+        if (!((a.mask >> v) & 1u)) continue;
+        double val = a.tmpl[(long)v * plane + jk];
+        int rank = a.idx[jk];
+        if (a.perturb && rank >= 0 && v >= 2 && v <= 4) {
+            const double *d = a.draws + 3 * ((long)gi * a.nact + rank);
+            if (v == 2) val = val + a.vamp * (2.0 * d[0] - 1.0);
+            else if (v == 3) val = val + a.vamp * (2.0 * d[1] - 1.0);
+            else val = val * (1.0 + a.pamp * (2.0 * d[2] - 1.0));
+        }
+        a.P[(long)v * g.ncell + c] = val;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_two_temperature(TwoTempArgs a) {
+#pragma clang fp contract(off)
+    const Grid &g = a.g;
+    long c = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= g.ncell) return;
+    int k = (int)(c % g.nz), j = (int)((c / g.nz) % g.ny), i = (int)(c / ((long)g.ny * g.nz));
+    if (i < a.ng || i >= g.nx - a.ng || j < a.ng || j >= g.ny - a.ng || k < a.ng || k >= g.nz - a.ng) return;
+    double rho = a.P[c], p = a.P[4 * g.ncell + c];
+    double T = p / pymax(rho, 1e-12);
+    double *Te = a.P + (long)a.off * g.ncell + c, *Ti = a.P + (long)(a.off + 1) * g.ncell + c;
+    double te = *Te, ti = *Ti;
+    te += a.dt * (T - te) / a.tau;
+    ti += a.dt * (T - ti) / a.tau;
+    *Te = te;
+    *Ti = ti;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_max_speed(MaxSpeedArgs a) {
+    const Grid &g = a.g;
+    __shared__ double red[BLOCK];
+    long ni = g.nx - 2, nj = g.ny - 2, nk = g.nz - 2, tot = ni * nj * nk;
+    double amax = 0.0;
+    Phys<double> ph;
+    ph.gamma = a.gamma;
+    ph.gfac = a.gfac;
+    for (long t = (long)blockIdx.x * BLOCK + threadIdx.x; t < tot; t += (long)gridDim.x * BLOCK) {
+        int k = 1 + (int)(t % nk), j = 1 + (int)((t / nk) % nj), i = 1 + (int)(t / (nj * nk));
+        long c = cidx(g, i, j, k);
+        double rho = a.P[c], vx = a.P[g.ncell + c], vy = a.P[2 * g.ncell + c], vz = a.P[3 * g.ncell + c],
+               p = a.P[4 * g.ncell + c];
+        double cs = srhd_sound_speed(ph, rho, p);
+        double l0, l1, l2, l3, l4, l5;
+        eig_speeds(vx, cs, l0, l1);
+        eig_speeds(vy, cs, l2, l3);
+        eig_speeds(vz, cs, l4, l5);
+        double loc = pymax(pymax(pymax(pymax(pymax(fabs(l0), fabs(l1)), fabs(l2)), fabs(l3)), fabs(l4)), fabs(l5));
+        if (loc > amax) amax = loc;
+    }
+    red[threadIdx.x] = amax;
+    __syncthreads();
+    for (int o = BLOCK / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = pymax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_reduce_max(const double *partial, int n, double floor_value,
+                                                       double *out) {
+    __shared__ double red[BLOCK];
+    double m = 0.0;
+    for (int t = threadIdx.x; t < n; t += BLOCK) m = pymax(m, partial[t]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = BLOCK / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = pymax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0] < floor_value ? floor_value : red[0];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_count_nonfinite(const double *x, long n, unsigned long long *partial) {
+    __shared__ unsigned long long red[BLOCK];
+    unsigned long long cnt = 0;
+    for (long t = (long)blockIdx.x * BLOCK + threadIdx.x; t < n; t += (long)gridDim.x * BLOCK)
+        cnt += isfinite(x[t]) ? 0ull : 1ull;
+    red[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int o = BLOCK / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_reduce_sum_u64(const unsigned long long *partial, int n,
+                                                           unsigned long long *out) {
+    __shared__ unsigned long long red[BLOCK];
+    unsigned long long m = 0;
+    for (int t = threadIdx.x; t < n; t += BLOCK) m += partial[t];
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = BLOCK / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+template <typename T> void launch_face_rmhd_t(const FaceLaunch &l, hipStream_t s);
+
+hipError_t launch_face_rmhd(const FaceLaunch &l, hipStream_t s) {
+    if (l.mixed) launch_face_rmhd_t<float>(l, s);
+    else launch_face_rmhd_t<double>(l, s);
+    return hipGetLastError();
+}
+
+template <int NB, typename F> static void upd(const UpdLaunch &l, bool rhs_only, hipStream_t s) {
+    UpdArgs<F> a;
+    static_assert(sizeof(UpdArgs<F>) == sizeof(UpdArgs<double>), "layout");
+    memcpy((void *)&a, (const void *)&l.a, sizeof(a));
+    dim3 grid = grid1d(l.a.g.ncell);
+    if (rhs_only) hipLaunchKernelGGL((k_rhs<NB, F>), grid, dim3(BLOCK), 0, s, a);
+    else hipLaunchKernelGGL((k_update<NB, F>), grid, dim3(BLOCK), 0, s, a);
+}
+
+hipError_t launch_update(const UpdLaunch &l, bool rhs_only, hipStream_t s) {
+    if (l.nb == 9) {
+        if (l.mixed) upd<9, float>(l, rhs_only, s);
+        else upd<9, double>(l, rhs_only, s);
+    } else {
+        if (l.mixed) upd<5, float>(l, rhs_only, s);
+        else upd<5, double>(l, rhs_only, s);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace afd

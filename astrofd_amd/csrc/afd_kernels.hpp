@@ -1,0 +1,516 @@
+// afd_kernels.hpp -- kernel templates of the SRHD/RMHD finite-volume step (gfx950).
+//
+// Data layout in HBM: every field is the reference's C-order array
+// (nv, nx+2ng, ny+2ng, nz+2ng) of float64, z fastest.  A stage of SSPRK runs as
+//   face kernels, one per direction: one thread per face between cells c-1 and c,
+//       lanes along z, writes the face flux at the index of cell c;
+//   k_update: one thread per cell (ghosts included): flux differences, GLM and
+//       resistive sources, the SSPRK combination in FP64 and FP64 con2prim.
+// For RMHD PPM/WENO the x face of cell c is evaluated twice (Q3 of the
+// reference): with its own HLL speeds (the right face of cell c-1) and with the
+// speeds carried from the z-back face of the cell that precedes c in i->j->k order
+// (the left face of cell c).  The z kernel runs first and stores those speeds.
+#pragma once
+#include "afd_device.hpp"
+
+#include <cstring>
+
+namespace afd {
+
+constexpr int BLOCK = 256;
+
+struct Grid {
+    int nv, nx, ny, nz;   // full dims (ghosts included)
+    int r0[3], r1[3];     // cells that receive a flux divergence: [r0, r1) per axis
+    long sx, sy, ncell;   // strides of x and y, number of cells
+};
+
+inline dim3 grid1d(long n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
+
+AFD_D long cidx(const Grid &g, int i, int j, int k) { return ((long)i * g.ny + j) * g.nz + k; }
+
+template <typename T> struct FaceArgs {
+    const double *P;   // primitives of the stage input
+    T *F;              // face flux (own speeds), nb x ncell
+    T *Fcar;           // RMHD PPM/WENO x only: flux with carried speeds
+    T *zs;             // RMHD PPM/WENO: z-back face speeds (sL, sR) per cell, 2 x ncell
+    double *aux;       // passives: (sL, sR, vL, vR) per face, 4 x ncell (or null)
+    Grid g;
+    Phys<T> ph;
+    double p_max, v_max;
+    int lim;
+};
+
+// ------------------------------------------------------------------ reconstruction
+// Face states of the face between cells c-1 and c along stride s, variable base pointer b.
+// The mixed path reconstructs deviations from q[c] in FP32 and adds them back in FP64.
+template <int RECON, typename T>
+AFD_D void recon_face(const double *__restrict__ b, long c, long s, int lim, double &qL, double &qR) {
+    constexpr bool mixed = sizeof(T) == 4;
+    const double ref = mixed ? b[c] : 0.0;
+    T l, r;
+    if (RECON == 0) {  // MUSCL: core/srhd_core.py:486-533
+        T qm2 = (T)(b[c - 2 * s] - ref), qm1 = (T)(b[c - s] - ref), q0 = (T)(b[c] - ref), qp1 = (T)(b[c + s] - ref);
+        T slL = limiter<T>(lim, qm1 - qm2, q0 - qm1);
+        T slR = limiter<T>(lim, q0 - qm1, qp1 - q0);
+        l = qm1 + T(0.5) * slL;
+        r = q0 - T(0.5) * slR;
+    } else {
+        T qm3 = (T)(b[c - 3 * s] - ref), qm2 = (T)(b[c - 2 * s] - ref), qm1 = (T)(b[c - s] - ref);
+        T q0 = (T)(b[c] - ref), qp1 = (T)(b[c + s] - ref), qp2 = (T)(b[c + 2 * s] - ref);
+        if (RECON == 1) {  // PPM: core/srhd_core.py:943-950
+            T a, bb;
+            ppm_reconstruct<T>(qm3, qm2, qm1, q0, qp1, a, l);
+            ppm_reconstruct<T>(qm2, qm1, q0, qp1, qp2, r, bb);
+        } else {  // WENO5: core/srhd_core.py:1183-1184
+            l = weno5_left<T>(qm3, qm2, qm1, q0, qp1);
+            r = weno5_right<T>(qp2, qp1, q0, qm1, qm2);
+        }
+    }
+    if (mixed) {
+        qL = ref + (double)l;
+        qR = ref + (double)r;
+    } else {
+        qL = (double)l;
+        qR = (double)r;
+    }
+}
+
+template <int DIR> AFD_D long dstride(const Grid &g) { return DIR == 0 ? g.sx : (DIR == 1 ? g.sy : 1L); }
+
+// Decompose a linear thread index over the face box of direction DIR
+// (faces [r0, r1] along DIR, cells [r0, r1) across it), z fastest.
+template <int DIR> AFD_D bool face_coords(const Grid &g, long t, int &i, int &j, int &k) {
+    int n[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) n[a] = g.r1[a] - g.r0[a] + (a == DIR ? 1 : 0);
+    long tot = (long)n[0] * n[1] * n[2];
+    if (t >= tot) return false;
+    k = (int)(t % n[2]);
+    long r = t / n[2];
+    j = (int)(r % n[1]);
+    i = (int)(r / n[1]);
+    i += g.r0[0];
+    j += g.r0[1];
+    k += g.r0[2];
+    return true;
+}
+
+// ------------------------------------------------------------------ SRHD face
+template <int RECON, int RIEM, int DIR, typename T>
+__global__ __launch_bounds__(BLOCK) void k_face_srhd(FaceArgs<T> a) {
+    const Grid &g = a.g;
+    int i, j, k;
+    if (!face_coords<DIR>(g, (long)blockIdx.x * BLOCK + threadIdx.x, i, j, k)) return;
+    const long c = cidx(g, i, j, k), s = dstride<DIR>(g);
+    double qLd[5], qRd[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) recon_face<RECON, T>(a.P + v * g.ncell, c, s, a.lim, qLd[v], qRd[v]);
+    srhd_floor(a.p_max, a.v_max, qLd);
+    srhd_floor(a.p_max, a.v_max, qRd);
+    T WL = (T)lorentz_guarded(qLd[1] * qLd[1] + qLd[2] * qLd[2] + qLd[3] * qLd[3]);
+    T WR = (T)lorentz_guarded(qRd[1] * qRd[1] + qRd[2] * qRd[2] + qRd[3] * qRd[3]);
+    T qL[5], qR[5], UL[5], UR[5], FL[5], FR[5], F[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        qL[v] = (T)qLd[v];
+        qR[v] = (T)qRd[v];
+    }
+    srhd_p2c_W(a.ph, qL, WL, UL);
+    srhd_p2c_W(a.ph, qR, WR, UR);
+    srhd_flux<DIR>(qL, UL, FL);
+    srhd_flux<DIR>(qR, UR, FR);
+    T csL = srhd_sound_speed(a.ph, qL[0], qL[4]), csR = srhd_sound_speed(a.ph, qR[0], qR[4]);
+    T lmL, lpL, lmR, lpR;
+    eig_speeds(qL[1 + DIR], csL, lmL, lpL);
+    eig_speeds(qR[1 + DIR], csR, lmR, lpR);
+    T sL = pymin(lmL, lmR), sR = pymax(lpL, lpR);
+    if (RIEM == 1)
+        hllc5<1 + DIR>(UL, UR, FL, FR, sL, sR, qL[4], qR[4], qL[1 + DIR], qR[1 + DIR], F);
+    else
+        hlle_vec<5>(UL, UR, FL, FR, sL, sR, F);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) a.F[v * g.ncell + c] = F[v];
+    if (a.aux) {
+        a.aux[c] = sL;
+        a.aux[g.ncell + c] = sR;
+        a.aux[2 * g.ncell + c] = qL[1 + DIR];
+        a.aux[3 * g.ncell + c] = qR[1 + DIR];
+    }
+}
+
+// ------------------------------------------------------------------ RMHD face
+template <int DIR, typename T> struct RmhdFace {
+    T UL[9], UR[9], FL[9], FR[9];
+    T pL, pR, vL, vR, sL, sR;
+};
+
+template <int RECON, int DIR, typename T>
+AFD_D void rmhd_face_setup(const FaceArgs<T> &a, long c, RmhdFace<DIR, T> &f) {
+    const Grid &g = a.g;
+    const long s = dstride<DIR>(g);
+    double qLd[9], qRd[9];
+#pragma unroll
+    for (int v = 0; v < 9; ++v) recon_face<RECON, T>(a.P + v * g.ncell, c, s, a.lim, qLd[v], qRd[v]);
+    rmhd_floor(a.p_max, a.v_max, qLd);
+    rmhd_floor(a.p_max, a.v_max, qRd);
+    double vcL[3], vcR[3];
+    double WLd = rmhd_cap_W(a.v_max, qLd, vcL), WRd = rmhd_cap_W(a.v_max, qRd, vcR);
+    // rmhd_fast_speed's own Lorentz factor differs only when the floored v2 still reaches V_MAX^2
+    double WfL = WLd, WfR = WRd;
+    if (qLd[1] * qLd[1] + qLd[2] * qLd[2] + qLd[3] * qLd[3] >= a.v_max * a.v_max) WfL = rmhd_fast_W(a.v_max, qLd);
+    if (qRd[1] * qRd[1] + qRd[2] * qRd[2] + qRd[3] * qRd[3] >= a.v_max * a.v_max) WfR = rmhd_fast_W(a.v_max, qRd);
+    T qL[9], qR[9], vL3[3], vR3[3];
+#pragma unroll
+    for (int v = 0; v < 9; ++v) {
+        qL[v] = (T)qLd[v];
+        qR[v] = (T)qRd[v];
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        vL3[m] = (T)vcL[m];
+        vR3[m] = (T)vcR[m];
+    }
+    rmhd_p2c_W(a.ph, qL, vL3, (T)WLd, f.UL);
+    rmhd_p2c_W(a.ph, qR, vR3, (T)WRd, f.UR);
+    rmhd_flux<DIR>(a.ph, qL, f.UL, f.FL);
+    rmhd_flux<DIR>(a.ph, qR, f.UR, f.FR);
+    T cfL = rmhd_fast_speed_W(a.ph, qL, (T)WfL), cfR = rmhd_fast_speed_W(a.ph, qR, (T)WfR);
+    T lmL, lpL, lmR, lpR;
+    eig_speeds(qL[1 + DIR], cfL, lmL, lpL);
+    eig_speeds(qR[1 + DIR], cfR, lmR, lpR);
+    f.sL = pymin(lmL, lmR);
+    f.sR = pymax(lpL, lpR);
+    f.pL = qL[4];
+    f.pR = qR[4];
+    f.vL = qL[1 + DIR];
+    f.vR = qR[1 + DIR];
+}
+
+// core/rmhd_core.py:569-578, 726-746 and the MUSCL-only hlld override (:811-831)
+template <int RECON, int RIEM, int DIR, typename T>
+AFD_D void rmhd_riemann(const Phys<T> &ph, const RmhdFace<DIR, T> &f, T sL, T sR, T *out) {
+    if (RIEM == 2) {
+        hlld_full<DIR>(ph, f.UL, f.UR, f.FL, f.FR, sL, sR, f.pL, f.pR, f.vL, f.vR, out);
+    } else if (RIEM == 1) {
+        hlle_vec<9>(f.UL, f.UR, f.FL, f.FR, sL, sR, out);
+        hllc5<1 + DIR>(f.UL, f.UR, f.FL, f.FR, sL, sR, f.pL, f.pR, f.vL, f.vR, out);
+        if (RECON == 0) {
+            T sM = hllc_contact_speed<1 + DIR>(f.UL, f.UR, sL, sR, f.pL, f.pR, f.vL, f.vR);
+            const T *Fm = (sM >= T(0)) ? f.FL : f.FR;
+#pragma unroll
+            for (int m = 5; m < 9; ++m) out[m] = Fm[m];
+        }
+    } else {
+        hlle_vec<9>(f.UL, f.UR, f.FL, f.FR, sL, sR, out);
+    }
+}
+
+template <int RECON, int RIEM, int DIR, typename T>
+__global__ __launch_bounds__(BLOCK) void k_face_rmhd(FaceArgs<T> a) {
+    const Grid &g = a.g;
+    int i, j, k;
+    if (!face_coords<DIR>(g, (long)blockIdx.x * BLOCK + threadIdx.x, i, j, k)) return;
+    const long c = cidx(g, i, j, k);
+    RmhdFace<DIR, T> f;
+    rmhd_face_setup<RECON, DIR, T>(a, c, f);
+    T out[9];
+    constexpr bool carried = (RECON != 0) && (DIR == 0);
+    if (!carried || i > g.r0[0]) {
+        rmhd_riemann<RECON, RIEM, DIR, T>(a.ph, f, f.sL, f.sR, out);
+#pragma unroll
+        for (int v = 0; v < 9; ++v) a.F[v * g.ncell + c] = out[v];
+    }
+    if (carried && i < g.r1[0]) {
+        // predecessor of cell (i,j,k) in the reference's i->j->k loop (core/rmhd_core.py:1447-1449)
+        T cL = T(-1), cR = T(1);
+        long pc = -1;
+        if (k > g.r0[2]) pc = c - 1;
+        else if (j > g.r0[1]) pc = cidx(g, i, j - 1, g.r1[2] - 1);
+        else if (i > g.r0[0]) pc = cidx(g, i - 1, g.r1[1] - 1, g.r1[2] - 1);
+        if (pc >= 0) {
+            cL = a.zs[pc];
+            cR = a.zs[g.ncell + pc];
+        }
+        rmhd_riemann<RECON, RIEM, DIR, T>(a.ph, f, cL, cR, out);
+#pragma unroll
+        for (int v = 0; v < 9; ++v) a.Fcar[v * g.ncell + c] = out[v];
+    }
+    if (DIR == 2 && a.zs) {
+        a.zs[c] = f.sL;
+        a.zs[g.ncell + c] = f.sR;
+    }
+    if (a.aux) {
+        a.aux[c] = f.sL;
+        a.aux[g.ncell + c] = f.sR;
+        a.aux[2 * g.ncell + c] = f.vL;
+        a.aux[3 * g.ncell + c] = f.vR;
+    }
+}
+
+// ------------------------------------------------------------------ passive scalars (Q15)
+// Passive fluxes of the reference's MUSCL (SRHD, RMHD) and PPM (SRHD) right-hand sides:
+// the face states of the WENO, PPM and MUSCL variants are recomputed here and combined with
+// the speeds/velocities the face kernels stored, in the reference's order of subtraction.
+struct PassiveArgs {
+    const double *P;
+    const double *aux[3];   // per direction (sL, sR, vL, vR) x ncell
+    double *rhs;            // n_passive x ncell, zero outside the range
+    Grid g;
+    int physics, recon, lim, np, po;
+    double d[3];
+};
+
+AFD_D double pwrap(const Grid &g, const double *P, int v, int i, int j, int k) {
+    // Python negative-index wrap of the reference's passive stencils (i-3 at i = 2);
+    // indices past the end wrap as well (the reference reads out of range there).
+    if (i < 0) i += g.nx;
+    if (j < 0) j += g.ny;
+    if (k < 0) k += g.nz;
+    if (i >= g.nx) i -= g.nx;
+    if (j >= g.ny) j -= g.ny;
+    if (k >= g.nz) k -= g.nz;
+    return P[v * g.ncell + cidx(g, i, j, k)];
+}
+
+// variant 0 WENO5, 1 PPM, 2 MUSCL (core/srhd_core.py:554-580) on the face (c-1 | c)
+AFD_D double passive_flux(const PassiveArgs &a, int variant, int v, int dir, int i, int j, int k, double vL,
+                          double vR, double sL, double sR) {
+    double q[6];
+    for (int o = -3; o <= 2; ++o)
+        q[o + 3] = pwrap(a.g, a.P, v, i + (dir == 0 ? o : 0), j + (dir == 1 ? o : 0), k + (dir == 2 ? o : 0));
+    double tL, tR;
+    if (variant == 0) {
+        tL = weno5_left<double>(q[0], q[1], q[2], q[3], q[4]);
+        tR = weno5_right<double>(q[5], q[4], q[3], q[2], q[1]);
+    } else if (variant == 1) {
+        double a0, b0;
+        ppm_reconstruct<double>(q[0], q[1], q[2], q[3], q[4], a0, tL);
+        ppm_reconstruct<double>(q[1], q[2], q[3], q[4], q[5], tR, b0);
+    } else {
+        double slL = limiter<double>(a.lim, q[2] - q[1], q[3] - q[2]);
+        double slR = limiter<double>(a.lim, q[3] - q[2], q[4] - q[3]);
+        tL = q[2] + 0.5 * slL;
+        tR = q[3] - 0.5 * slR;
+    }
+    return hlle_scalar<double>(tL, tR, tL * vL, tR * vR, sL, sR);
+}
+
+__global__ void k_passive_rhs(PassiveArgs a);
+
+// ------------------------------------------------------------------ stage update + c2p
+template <typename F> struct UpdArgs {
+    const double *P0;     // step input primitives (U0 at stage 0, passives t0)
+    const double *Pin;    // stage input primitives (GLM, resistivity, passives t_{s-1})
+    double *Pout;         // stage output primitives
+    double *U0, *Uc;      // conserved: step start, running stage value (FP64)
+    const F *Fx, *FxL, *Fy, *Fz;  // face fluxes; FxL is the left-face x flux (carried speeds)
+    const double *rhs_pass;       // passive rhs (or null = 0)
+    unsigned long long *stats;    // [hydro_iters, newton_evals, fallback, bisection, invalid, learned] or null
+    Grid g;
+    int stage, nst, n_resist, np, po;
+    double dt, dx, dy, dz, glm_cp2, resistivity;
+    double gamma, gfac, p_max, v_max;
+    const void *mlp;              // learned-guess weights (null: plain con2prim)
+};
+
+AFD_D double ssprk_combine(int stage, int nst, double u0, double uc, double r) {
+#pragma clang fp contract(off)
+    if (stage == 0) return u0 + r;
+    if (nst == 2) return 0.5 * (u0 + uc + r);
+    if (stage == 1) return 0.75 * u0 + 0.25 * (uc + r);
+    return (1.0 / 3.0) * u0 + (2.0 / 3.0) * (uc + r);
+}
+
+AFD_D void wave_add_stats(unsigned long long *stats, const C2PStats &st) {
+    unsigned long long v[6] = {(unsigned long long)st.hydro_iters, (unsigned long long)st.newton_iters,
+                               (unsigned long long)(st.flags & 1), (unsigned long long)((st.flags >> 1) & 1),
+                               (unsigned long long)((st.flags >> 2) & 1), (unsigned long long)((st.flags >> 3) & 1)};
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+        unsigned long long x = v[m];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+        if ((threadIdx.x & 63) == 0 && x) atomicAdd(&stats[m], x);
+    }
+}
+
+// Flux divergence and sources of one cell, in the reference's order of accumulation:
+// x, y, z differences (core/srhd_core.py:662, 779, 922), then GLM damping and the resistive
+// Laplacian of B (core/rmhd_core.py:1264-1275), n_resist times.
+template <int NB, typename F> AFD_D void cell_rhs(const UpdArgs<F> &a, long c, bool in, double *rhs) {
+#pragma clang fp contract(off)
+    const Grid &g = a.g;
+#pragma unroll
+    for (int v = 0; v < NB; ++v) rhs[v] = 0.0;
+    if (!in) return;
+#pragma unroll
+    for (int v = 0; v < NB; ++v) {
+        long o = v * g.ncell + c;
+        rhs[v] -= ((double)a.Fx[o + g.sx] - (double)a.FxL[o]) / a.dx;
+        rhs[v] -= ((double)a.Fy[o + g.sy] - (double)a.Fy[o]) / a.dy;
+        rhs[v] -= ((double)a.Fz[o + 1] - (double)a.Fz[o]) / a.dz;
+    }
+    if (NB == 9) {
+        rhs[8 % NB] -= a.glm_cp2 * a.Pin[8 * g.ncell + c];
+        double invdx2 = 1.0 / (a.dx * a.dx + SMALL), invdy2 = 1.0 / (a.dy * a.dy + SMALL),
+               invdz2 = 1.0 / (a.dz * a.dz + SMALL);
+        for (int rep = 0; rep < a.n_resist; ++rep) {
+#pragma unroll
+            for (int cc = 5; cc < 8; ++cc) {
+                const double *q = a.Pin + cc * g.ncell;
+                double lap = (q[c + g.sx] - 2.0 * q[c] + q[c - g.sx]) * invdx2 +
+                             (q[c + g.sy] - 2.0 * q[c] + q[c - g.sy]) * invdy2 +
+                             (q[c + 1] - 2.0 * q[c] + q[c - 1]) * invdz2;
+                rhs[cc % NB] += a.resistivity * lap;
+            }
+        }
+    }
+}
+
+AFD_D bool in_range(const Grid &g, long c) {
+    int k = (int)(c % g.nz), j = (int)((c / g.nz) % g.ny), i = (int)(c / ((long)g.ny * g.nz));
+    return i >= g.r0[0] && i < g.r1[0] && j >= g.r0[1] && j < g.r1[1] && k >= g.r0[2] && k < g.r1[2];
+}
+
+// rhs of the stage input only (the reference's compute_rhs_*), written to Pout as (nb + np) x ncell
+template <int NB, typename F>
+__global__ __launch_bounds__(BLOCK) void k_rhs(UpdArgs<F> a) {
+    const Grid &g = a.g;
+    long c = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= g.ncell) return;
+    double rhs[NB];
+    cell_rhs<NB, F>(a, c, in_range(g, c), rhs);
+#pragma unroll
+    for (int v = 0; v < NB; ++v) a.Pout[v * g.ncell + c] = rhs[v];
+    for (int t = 0; t < a.np; ++t) a.Pout[(NB + t) * g.ncell + c] = a.rhs_pass ? a.rhs_pass[t * g.ncell + c] : 0.0;
+}
+
+template <int NB, typename F>
+__global__ __launch_bounds__(BLOCK) void k_update(UpdArgs<F> a) {
+    const Grid &g = a.g;
+    long c = (long)blockIdx.x * BLOCK + threadIdx.x;
+    C2PStats st = {0, 0, 0};
+    if (c < g.ncell) {
+        double rhs[NB];
+        cell_rhs<NB, F>(a, c, in_range(g, c), rhs);
+        double U[NB], q[NB];
+        if (a.stage == 0) {
+            double q0[NB];
+#pragma unroll
+            for (int v = 0; v < NB; ++v) q0[v] = a.P0[v * g.ncell + c];
+            if (NB == 9) rmhd_p2c_cell(a.gfac, a.p_max, a.v_max, q0, U);
+            else srhd_p2c_cell(a.gfac, q0, U);
+#pragma unroll
+            for (int v = 0; v < NB; ++v) a.U0[v * g.ncell + c] = U[v];
+        } else {
+#pragma unroll
+            for (int v = 0; v < NB; ++v) U[v] = a.U0[v * g.ncell + c];
+        }
+#pragma unroll
+        for (int v = 0; v < NB; ++v) {
+            double r;
+            {
+#pragma clang fp contract(off)
+                r = a.dt * rhs[v];
+            }
+            double uc = (a.stage == 0) ? 0.0 : a.Uc[v * g.ncell + c];
+            U[v] = ssprk_combine(a.stage, a.nst, U[v], uc, r);
+            a.Uc[v * g.ncell + c] = U[v];
+        }
+        if (NB == 9) {
+            RmhdC2PParams cp = {a.gamma, a.gfac, a.p_max, a.v_max};
+            rmhd_c2p(cp, U, q, st);
+        } else {
+            hydro_c2p(a.gamma, a.gfac, a.p_max, a.v_max, U, q, st);
+        }
+#pragma unroll
+        for (int v = 0; v < NB; ++v) a.Pout[v * g.ncell + c] = q[v];
+        for (int t = 0; t < a.np; ++t) {
+            long o = (long)(a.po + t) * g.ncell + c;
+            double rt = a.rhs_pass ? a.rhs_pass[t * g.ncell + c] : 0.0;
+            double r;
+            {
+#pragma clang fp contract(off)
+                r = a.dt * rt;
+            }
+            a.Pout[o] = ssprk_combine(a.stage, a.nst, a.P0[o], a.Pin[o], r);
+        }
+    }
+    if (a.stats) wave_add_stats(a.stats, st);
+}
+
+// ------------------------------------------------------------------ pointwise batches (test hooks)
+struct BatchArgs {
+    const double *in;
+    double *out;
+    int *iters;  // per state: hydro iterations + Newton evaluations (or null)
+    long n;
+    int physics;
+    double gamma, gfac, p_max, v_max;
+};
+
+__global__ void k_c2p_batch(BatchArgs a);
+
+__global__ void k_p2c_batch(BatchArgs a);
+
+// ------------------------------------------------------------------ boundary fills
+// core/boundary.py:4-8: y ghosts (all x, all z) from the opposite interior planes, then z
+// ghosts (all x, all y) the same way.  bc = 0 periodic, 1 zero-gradient (additive option).
+struct BcArgs {
+    double *P;
+    Grid g;
+    int ng, bc;
+};
+
+__global__ void k_fill_y(BcArgs a);
+
+__global__ void k_fill_z(BcArgs a);
+
+// core/boundary.py:10-11: every +x ghost plane from the last interior plane
+__global__ void k_outflow_x(BcArgs a);
+
+// core/nozzle.py:27-92: the -x ghost planes over interior (j, k).  The host precomputes the
+// profile of every variable the reference writes (nozzle_weight uses libm tanh/cos); the
+// perturbation takes 3 uniform draws per cell with s > 0, in g -> j -> k order.
+struct NozzleArgs {
+    double *P;
+    const double *tmpl;   // nv x ny_int x nz_int
+    const int *idx;       // ny_int x nz_int: rank among cells with s > 0, or -1
+    const double *draws;  // 3 * ng * nact uniforms in [0, 1)
+    Grid g;
+    int ng, ny_int, nz_int, nact, perturb;
+    unsigned mask;        // bit v set: the nozzle writes variable v
+    double vamp, pamp;
+};
+
+__global__ void k_nozzle(NozzleArgs a);
+
+// core/source_terms.py:30-55 (interior cells)
+struct TwoTempArgs {
+    double *P;
+    Grid g;
+    int ng, off;
+    double dt, tau;
+};
+__global__ void k_two_temperature(TwoTempArgs a);
+
+// ------------------------------------------------------------------ reductions
+// core/srhd_core.py:1267-1281 over cells [1, n-1)^3; one partial maximum per block
+struct MaxSpeedArgs {
+    const double *P;
+    double *partial;
+    Grid g;
+    double gamma, gfac;
+};
+__global__ void k_max_speed(MaxSpeedArgs a);
+
+__global__ void k_reduce_max(const double *partial, int n, double floor_value,
+                                                       double *out);
+
+// count of non-finite values over n doubles; one partial count per block
+__global__ void k_count_nonfinite(const double *x, long n, unsigned long long *partial);
+
+__global__ void k_reduce_sum_u64(const unsigned long long *partial, int n,
+                                                           unsigned long long *out);
+
+}  // namespace afd

@@ -1,0 +1,57 @@
+// afd_launch.hpp -- host-side launchers of the gfx950 kernels (one per kernel family).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "afd_kernels.hpp"
+
+namespace afd {
+
+// Arguments of a face-kernel launch in precision-neutral form.
+struct FaceLaunch {
+    const double *P;
+    void *F, *Fcar, *zs;  // flux type: double (plain) or float (mixed)
+    double *aux;
+    Grid g;
+    Phys<double> phd;
+    Phys<float> phf;
+    double p_max, v_max;
+    int lim, recon, riemann, dir;
+    bool mixed;
+};
+
+template <typename T> inline FaceArgs<T> face_args(const FaceLaunch &l) {
+    FaceArgs<T> a;
+    a.P = l.P;
+    a.F = (T *)l.F;
+    a.Fcar = (T *)l.Fcar;
+    a.zs = (T *)l.zs;
+    a.aux = l.aux;
+    a.g = l.g;
+    a.p_max = l.p_max;
+    a.v_max = l.v_max;
+    a.lim = l.lim;
+    if constexpr (sizeof(T) == 8) a.ph = l.phd;
+    else a.ph = l.phf;
+    return a;
+}
+
+// Number of faces (threads) of the face box of direction dir.
+inline long face_count(const Grid &g, int dir) {
+    long n = 1;
+    for (int a = 0; a < 3; ++a) n *= (long)(g.r1[a] - g.r0[a] + (a == dir ? 1 : 0));
+    return n;
+}
+
+hipError_t launch_face_srhd(const FaceLaunch &l, hipStream_t s);
+hipError_t launch_face_rmhd(const FaceLaunch &l, hipStream_t s);
+
+// Stage update / rhs-only launch.  flux pointers typed by `mixed`.
+struct UpdLaunch {
+    UpdArgs<double> a;  // flux pointers reinterpreted as float when mixed
+    bool mixed;
+    int nb;
+};
+hipError_t launch_update(const UpdLaunch &l, bool rhs_only, hipStream_t s);
+
+
+}  // namespace afd

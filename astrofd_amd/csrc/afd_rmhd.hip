@@ -1,0 +1,37 @@
+// afd_rmhd.hip -- RMHD face-kernel instantiations (MUSCL/PPM/WENO x HLLE/hlld/hlld_full x x,y,z).
+// The precision is selected at build time by AFD_RMHD_T (double or float) so the two
+// halves compile as separate translation units in parallel.
+#include "afd_launch.hpp"
+
+#ifndef AFD_RMHD_T
+#define AFD_RMHD_T double
+#endif
+
+namespace afd {
+
+template <int RECON, int RIEM, typename T> static void go(const FaceLaunch &l, hipStream_t s) {
+    FaceArgs<T> a = face_args<T>(l);
+    dim3 grid = grid1d(face_count(l.g, l.dir));
+    if (l.dir == 0) hipLaunchKernelGGL((k_face_rmhd<RECON, RIEM, 0, T>), grid, dim3(BLOCK), 0, s, a);
+    else if (l.dir == 1) hipLaunchKernelGGL((k_face_rmhd<RECON, RIEM, 1, T>), grid, dim3(BLOCK), 0, s, a);
+    else hipLaunchKernelGGL((k_face_rmhd<RECON, RIEM, 2, T>), grid, dim3(BLOCK), 0, s, a);
+}
+
+template <typename T> void launch_face_rmhd_t(const FaceLaunch &l, hipStream_t s) {
+    int key = l.recon * 3 + (l.riemann < 0 ? 0 : (l.riemann > 2 ? 2 : l.riemann));
+    switch (key) {
+        case 0: go<0, 0, T>(l, s); break;
+        case 1: go<0, 1, T>(l, s); break;
+        case 2: go<0, 2, T>(l, s); break;
+        case 3: go<1, 0, T>(l, s); break;
+        case 4: go<1, 1, T>(l, s); break;
+        case 5: go<1, 2, T>(l, s); break;
+        case 6: go<2, 0, T>(l, s); break;
+        case 7: go<2, 1, T>(l, s); break;
+        default: go<2, 2, T>(l, s); break;
+    }
+}
+
+template void launch_face_rmhd_t<AFD_RMHD_T>(const FaceLaunch &l, hipStream_t s);
+
+}  // namespace afd

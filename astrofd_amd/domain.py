@@ -1,0 +1,271 @@
+"""Device-resident domain: one libafd context holding the primitive state on the GPU.
+
+`params_from_settings` restates what the reference's configure() functions
+derive from the settings dict (core/srhd_core.py:18-41, core/rmhd_core.py:21-50)
+and what the driver passes per call (grid spacing, RK order, NG).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+LIMITERS = {"minmod": 1, "vanleer": 2}  # anything else -> mc (0)
+
+
+def limiter_id(name):
+    return LIMITERS.get(str(name).lower(), 0)
+
+
+def recon_id(name):
+    name = str(name).lower()
+    return 1 if name == "ppm" else (2 if name == "weno" else 0)
+
+
+def riemann_id(physics, name):
+    name = str(name).lower()
+    if physics == "rmhd":  # core/rmhd_core.py:42-46: "hllc" falls through to HLLE
+        return 2 if name == "hlld_full" else (1 if name == "hlld" else 0)
+    return 1 if name == "hllc" else 0  # core/srhd_core.py:38-39
+
+
+def n_resist_for(s, recon):
+    """Applications of the resistive Laplacian per RHS.  Reference: once in MUSCL
+    (core/rmhd_core.py:1265-1275), twice in PPM (:1414-1435), never in WENO (:1555)."""
+    if not (s.get("RESISTIVE_ENABLED", False) and float(s.get("RESISTIVITY", 0.0)) > 0.0):
+        return 0
+    if str(s.get("RESISTIVE_MODE", "reference")).lower() == "once":
+        return 1
+    return {0: 1, 1: 2, 2: 0}[recon]
+
+
+def params_from_settings(s, device=0, **override):
+    phys = str(s.get("PHYSICS", "hydro")).lower()
+    if phys not in ("hydro", "rmhd"):
+        raise NotImplementedError(f"PHYSICS={phys!r}: only the SRHD (hydro) and RMHD step are on this path")
+    for key in ("DISSIPATION_ENABLED", "CHEMISTRY_ENABLED", "COOLING_ENABLED"):
+        if s.get(key, False):
+            raise NotImplementedError(f"{key}: the post-step operator is not on this path")
+    rm = phys == "rmhd"
+    nb = 9 if rm else 5
+    rc = recon_id(s.get("RECON", "muscl"))
+    p = L.Params()
+    p.physics = L.AFD_PHYS_RMHD if rm else L.AFD_PHYS_HYDRO
+    p.n_passive = int(s.get("N_PASSIVE", 0))
+    p.passive_offset = int(s.get("PASSIVE_OFFSET", nb))
+    p.nv = nb + p.n_passive
+    p.nx, p.ny, p.nz, p.ng = int(s["NX"]), int(s["NY"]), int(s["NZ"]), int(s["NG"])
+    p.dx = s["Lx"] / s["NX"]
+    p.dy = s["Ly"] / s["NY"]
+    p.dz = s["Lz"] / s["NZ"]
+    # The SRHD helpers bind gamma=GAMMA at definition time (core/srhd_core.py:47, 62, 134), so the
+    # reference SRHD step always computes with 5/3 whatever the config says; RMHD passes GAMMA.
+    p.gamma = float(s.get("GAMMA", 5.0 / 3.0)) if rm else 5.0 / 3.0
+    p.p_max = float(s.get("P_MAX", 1.0))
+    p.v_max = float(s.get("V_MAX", 0.999))
+    p.glm_ch = float(s.get("GLM_CH", 1.0))
+    p.glm_cp = float(s.get("GLM_CP", 0.1))
+    p.limiter_id = limiter_id(s.get("LIMITER", "mc"))
+    p.recon_id = rc
+    p.riemann_id = riemann_id(phys, s.get("RIEMANN", "hlle"))
+    p.rk_order = 3 if int(s.get("RK_ORDER", 2)) == 3 else 2
+    p.n_resist = n_resist_for(s, rc) if rm else 0
+    p.resistivity = float(s.get("RESISTIVITY", 0.0))
+    p.precision = L.AFD_PREC_MIXED if str(s.get("PRECISION", "fp64")).lower() == "mixed" else L.AFD_PREC_FP64
+    p.c2p_guess = L.AFD_C2P_LEARNED if str(s.get("C2P_GUESS", "plain")).lower() == "learned" else L.AFD_C2P_PLAIN
+    p.bc_y = L.AFD_BC_OUTFLOW if str(s.get("BC_Y", "periodic")).lower() == "outflow" else L.AFD_BC_PERIODIC
+    p.bc_z = L.AFD_BC_OUTFLOW if str(s.get("BC_Z", "periodic")).lower() == "outflow" else L.AFD_BC_PERIODIC
+    p.two_temperature = int(bool(s.get("TWO_TEMPERATURE", False)))
+    p.thermo_offset = int(s.get("THERMO_OFFSET", nb))
+    p.tei_tau = float(s.get("TEI_TAU", 0.5))
+    p.device = device
+    for k, v in override.items():
+        setattr(p, k, v)
+    return p
+
+
+PROFILES = {"top_hat": 0, "taper": 1, "smooth": 1, "parabolic": 2, "parabola": 2}
+B_INITS = {"poloidal": 1, "toroidal": 2}
+
+
+def nozzle_from_settings(s):
+    """The nozzle parameters core/nozzle.py:27-92 reads from cfg (plus the additive B_AMB)."""
+    n = L.Nozzle()
+    jc = s["JET_CENTER"]
+    n.y0, n.z0 = float(jc[1]), float(jc[2])
+    n.jet_radius = float(s["JET_RADIUS"])
+    n.shear_thick = float(s["SHEAR_THICK"])
+    n.profile = PROFILES.get(str(s.get("NOZZLE_PROFILE", "top_hat")).lower(), 0)
+    n.eta_rho, n.rho_amb, n.p_eq = float(s["ETA_RHO"]), float(s["RHO_AMB"]), float(s["P_EQ"])
+    n.gamma_jet = float(s["GAMMA_JET"])
+    n.vx_amb, n.vy_amb, n.vz_amb = float(s["VX_AMB"]), float(s["VY_AMB"]), float(s["VZ_AMB"])
+    n.perturb = 0 if str(s.get("NOZZLE_PERTURB", "none")).lower() == "none" else 1
+    n.turb_vamp, n.turb_pamp = float(s.get("TURB_VAMP", 0.0)), float(s.get("TURB_PAMP", 0.0))
+    ntr = int(s.get("N_TRACERS", 0))
+    if ntr > 8:
+        raise ValueError("at most 8 tracers")
+    n.n_tracers, n.tracer_offset = ntr, int(s.get("TRACER_OFFSET", 5))
+    tn = list(s.get("TRACER_NOZZLE_VALUES") or [])
+    ta = list(s.get("TRACER_AMB_VALUES") or [])
+    for t in range(ntr):  # core/nozzle.py:78-79
+        n.tracer_nozzle[t] = tn[t] if t < len(tn) else (1.0 if t == 0 else 0.0)
+        n.tracer_amb[t] = ta[t] if t < len(ta) else 0.0
+    n.two_temperature = int(bool(s.get("TWO_TEMPERATURE", False)))
+    n.thermo_offset = int(s.get("THERMO_OFFSET", 0))
+    te = s.get("TE_NOZZLE", s.get("TE_AMB", 0.0))
+    ti = s.get("TI_NOZZLE", s.get("TI_AMB", 0.0))
+    n.te_nozzle = float(te if te is not None else 0.0)
+    n.ti_nozzle = float(ti if ti is not None else 0.0)
+    n.b_init = B_INITS.get(str(s.get("B_INIT", "none")).lower(), 0)
+    n.b0 = float(s.get("B0", 0.0))
+    bamb = s.get("B_AMB") or [0.0, 0.0, 0.0]
+    for m in range(3):
+        n.b_amb[m] = float(bamb[m])
+    return n
+
+
+class Domain:
+    """One device context: the primitive array `pr` of the reference, resident in HBM."""
+
+    def __init__(self, params):
+        self.lib = L.load()
+        self.params = params
+        self.nb = 9 if params.physics == L.AFD_PHYS_RMHD else 5
+        self.shape = (params.nv, params.nx + 2 * params.ng, params.ny + 2 * params.ng, params.nz + 2 * params.ng)
+        self.size = int(np.prod(self.shape))
+        h = ctypes.c_void_p()
+        L.check(self.lib.afd_create(ctypes.byref(params), ctypes.byref(h)), "afd_create")
+        self.h = h
+        self.nozzle_set = False
+
+    @classmethod
+    def from_settings(cls, s, device=0, **override):
+        return cls(params_from_settings(s, device, **override))
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.afd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- state
+    def set_prim(self, pr):
+        pr = L.f64c(pr)
+        if pr.shape != self.shape:
+            raise ValueError(f"pr shape {pr.shape} != {self.shape}")
+        L.check(self.lib.afd_set_prim(self.h, L.dptr(pr), pr.size), "afd_set_prim")
+
+    def get_prim(self):
+        out = np.empty(self.shape)
+        L.check(self.lib.afd_get_prim(self.h, L.dptr(out), out.size), "afd_get_prim")
+        return out
+
+    # ---- boundaries
+    def set_nozzle(self, s):
+        n = nozzle_from_settings(s)
+        L.check(self.lib.afd_set_nozzle(self.h, ctypes.byref(n)), "afd_set_nozzle")
+        self.nozzle_set = True
+
+    def nozzle_draw_count(self):
+        n = ctypes.c_int64()
+        L.check(self.lib.afd_nozzle_draw_count(self.h, ctypes.byref(n)), "afd_nozzle_draw_count")
+        return n.value
+
+    def _draws(self, rng):
+        n = self.nozzle_draw_count() if self.nozzle_set else 0
+        if n == 0:
+            return None, 0
+        if rng is None:
+            raise ValueError("a perturbed nozzle needs the numpy Generator of the run")
+        return L.f64c(rng.random(n)), n
+
+    def apply_bcs(self, rng=None):
+        d, n = self._draws(rng)
+        L.check(self.lib.afd_apply_bcs(self.h, L.dptr(d) if d is not None else None, n), "afd_apply_bcs")
+
+    def apply_periodic_yz(self):
+        L.check(self.lib.afd_apply_periodic_yz(self.h), "afd_apply_periodic_yz")
+
+    def apply_outflow_right_x(self):
+        L.check(self.lib.afd_apply_outflow_right_x(self.h), "afd_apply_outflow_right_x")
+
+    def apply_nozzle_left_x(self, rng=None):
+        d, n = self._draws(rng)
+        L.check(self.lib.afd_apply_nozzle_left_x(self.h, L.dptr(d) if d is not None else None, n),
+                "afd_apply_nozzle_left_x")
+
+    # ---- operators
+    def max_char_speed(self):
+        a = ctypes.c_double()
+        L.check(self.lib.afd_max_char_speed(self.h, ctypes.byref(a)), "afd_max_char_speed")
+        return a.value
+
+    def step(self, dt):
+        L.check(self.lib.afd_step(self.h, float(dt)), "afd_step")
+
+    def compute_rhs(self):
+        out = np.empty((self.nb + self.params.n_passive,) + self.shape[1:])
+        L.check(self.lib.afd_compute_rhs(self.h, L.dptr(out), out.size), "afd_compute_rhs")
+        return out
+
+    def two_temperature(self, dt):
+        L.check(self.lib.afd_two_temperature(self.h, float(dt)), "afd_two_temperature")
+
+    def count_nonfinite(self):
+        n = ctypes.c_int64()
+        L.check(self.lib.afd_count_nonfinite(self.h, ctypes.byref(n)), "afd_count_nonfinite")
+        return n.value
+
+    def p2c(self, prim):
+        prim = L.f64c(prim).reshape(-1, self.nb)
+        out = np.empty_like(prim)
+        L.check(self.lib.afd_p2c_batch(self.h, prim.shape[0], L.dptr(prim), L.dptr(out)), "afd_p2c_batch")
+        return out
+
+    def c2p(self, cons, with_iters=False):
+        cons = L.f64c(cons).reshape(-1, self.nb)
+        out = np.empty_like(cons)
+        it = np.zeros(cons.shape[0], dtype=np.int32)
+        L.check(self.lib.afd_c2p_batch(self.h, cons.shape[0], L.dptr(cons), L.dptr(out),
+                                       it.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))), "afd_c2p_batch")
+        return (out, it) if with_iters else out
+
+    # ---- instrumentation
+    def set_instrumentation(self, stats=True, timing=False):
+        L.check(self.lib.afd_set_instrumentation(self.h, int(stats), int(timing)), "afd_set_instrumentation")
+
+    def stats(self):
+        st = L.Stats()
+        L.check(self.lib.afd_get_stats(self.h, ctypes.byref(st)), "afd_get_stats")
+        return st.as_dict()
+
+    def reset_stats(self):
+        L.check(self.lib.afd_reset_stats(self.h), "afd_reset_stats")
+
+    def synchronize(self):
+        L.check(self.lib.afd_synchronize(self.h), "afd_synchronize")
+
+    # ---- learned guess
+    def set_mlp_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        L.check(self.lib.afd_mlp_set_weights(self.h, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), w.size),
+                "afd_mlp_set_weights")
+
+    def get_mlp_weights(self):
+        n = ctypes.c_int32()
+        L.check(self.lib.afd_mlp_param_count(ctypes.byref(n)), "afd_mlp_param_count")
+        w = np.empty(n.value, dtype=np.float32)
+        L.check(self.lib.afd_mlp_get_weights(self.h, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), w.size),
+                "afd_mlp_get_weights")
+        return w

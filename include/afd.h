@@ -1,0 +1,173 @@
+/*
+ * afd.h -- C-ABI of the MI355X (gfx950) SRHD/RMHD finite-volume step.
+ *
+ * The library libafd.so replaces the hot path of AstroFD: the SRHD and RMHD
+ * cores (core/srhd_core.py, core/rmhd_core.py), the ghost-zone fills
+ * (core/boundary.py), the nozzle inflow (core/nozzle.py) and the
+ * two-temperature source (core/source_terms.py:30-55).  The reference is a
+ * Python module-function API; a maintainer binds this ABI with ctypes (the
+ * binding shipped in astrofd_amd/_lib.py).
+ *
+ * Conventions
+ *   - Every entry point returns int: 0 on success, < 0 on error (AFD_ERR_*);
+ *     afd_last_error() gives the message of the calling thread's last error.
+ *     Numerical trouble is clamped as in the reference, never an error.
+ *   - State arrays are float64 in the reference layout: C-order
+ *     (nv, nx + 2 ng, ny + 2 ng, nz + 2 ng), z fastest, ghost zones included.
+ *     The caller owns host arrays; device memory belongs to the context.
+ *   - One context = one device + one HIP stream (default priority).  Calls on
+ *     one context are not reentrant.
+ */
+#ifndef AFD_H
+#define AFD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AFD_OK 0
+#define AFD_ERR_INVALID (-1) /* bad argument or size */
+#define AFD_ERR_NOMEM (-2)   /* device allocation failed */
+#define AFD_ERR_HIP (-3)     /* HIP runtime error */
+#define AFD_ERR_STATE (-4)   /* call not valid in the context's state */
+
+#define AFD_PHYS_HYDRO 0
+#define AFD_PHYS_RMHD 1
+#define AFD_RECON_MUSCL 0
+#define AFD_RECON_PPM 1
+#define AFD_RECON_WENO 2
+#define AFD_PREC_FP64 0  /* everything in FP64 */
+#define AFD_PREC_MIXED 1 /* reconstruction + Riemann in FP32; Lorentz factor, con2prim, SSPRK in FP64 */
+#define AFD_C2P_PLAIN 0
+#define AFD_C2P_LEARNED 1 /* MLP initial guess for the RMHD Newton solve; the plain solve judges and falls back */
+#define AFD_BC_PERIODIC 0
+#define AFD_BC_OUTFLOW 1
+
+/* Everything the reference's configure() reads (core/srhd_core.py:18-41,
+ * core/rmhd_core.py:21-50) plus the grid.  Limiter/recon/Riemann IDs follow the
+ * reference: LIMITER mc=0 minmod=1 vanleer=2; SRHD RIEMANN hlle=0 hllc=1;
+ * RMHD RIEMANN hlle=0 hlld=1 (HLLC hydro + HLLE B) hlld_full=2. */
+typedef struct afd_params {
+    int32_t physics;          /* AFD_PHYS_* */
+    int32_t nv;               /* variables incl. passives: 5 or 9, + n_passive */
+    int32_t nx, ny, nz;       /* interior cells */
+    int32_t ng;               /* ghost layers (>= 2) */
+    double dx, dy, dz;
+    double gamma;             /* SRHD computes with the 5/3 bound at definition time in the reference */
+    double p_max, v_max;
+    double glm_ch, glm_cp;
+    int32_t limiter_id, recon_id, riemann_id, rk_order;
+    int32_t n_passive, passive_offset;
+    int32_t n_resist;         /* applications of the resistive Laplacian per RHS (reference: 1 MUSCL, 2 PPM, 0 WENO) */
+    double resistivity;
+    int32_t precision;        /* AFD_PREC_* */
+    int32_t c2p_guess;        /* AFD_C2P_* (RMHD only) */
+    int32_t bc_y, bc_z;       /* AFD_BC_*; the reference is periodic in y and z */
+    int32_t two_temperature, thermo_offset;
+    double tei_tau;
+    int32_t device;           /* HIP device ordinal */
+} afd_params;
+
+/* Nozzle inflow on the -x ghost planes (core/nozzle.py:27-92).  The library's host code
+ * evaluates nozzle_weight (core/nozzle.py:5-25) per (j, k) and uploads the profile. */
+typedef struct afd_nozzle {
+    double y0, z0;            /* jet centre (JET_CENTER[1], JET_CENTER[2]) */
+    double jet_radius, shear_thick;
+    int32_t profile;          /* 0 top_hat, 1 taper/smooth, 2 parabolic/parabola */
+    double eta_rho, rho_amb, p_eq, gamma_jet, vx_amb, vy_amb, vz_amb;
+    int32_t perturb;          /* 0: NOZZLE_PERTURB == "none" */
+    double turb_vamp, turb_pamp;
+    int32_t n_tracers, tracer_offset;
+    double tracer_nozzle[8], tracer_amb[8];
+    int32_t two_temperature, thermo_offset;
+    double te_nozzle, ti_nozzle;
+    int32_t b_init;           /* 0 none, 1 poloidal (Bx = B0 s), 2 toroidal (By = B0 s) */
+    double b0;
+    double b_amb[3];          /* additive ambient field added to the nozzle B (zeros: reference) */
+} afd_nozzle;
+
+typedef struct afd_stats {
+    uint64_t hydro_iters;     /* fixed-point iterations of the hydro (guess) solve */
+    uint64_t newton_evals;    /* f(Z) evaluations of the RMHD Newton stage */
+    uint64_t hydro_fallbacks; /* cells where the fixed point took the energy fallback */
+    uint64_t bisections;      /* RMHD cells that reached bisection */
+    uint64_t invalid;         /* RMHD cells whose recovery was invalid (hydro guess kept) */
+    uint64_t learned_ok;      /* RMHD cells solved from the learned guess (judge passed) */
+    uint64_t c2p_cells;       /* cells converted */
+    double ms_face[3];        /* summed HIP-event time of the x, y, z face kernels (timing on) */
+    double ms_update;         /* summed time of the stage-update (+ con2prim) kernel */
+    double ms_other;          /* BCs, passives, reductions */
+    uint64_t launches_face[3], launches_update;
+} afd_stats;
+
+typedef struct afd_ctx afd_ctx;
+
+/* Context: replaces configure(params) (core/srhd_core.py:18, core/rmhd_core.py:21)
+ * and the module globals it sets, plus init_block's allocation
+ * (solvers/srhd3d_mpi_muscl.py:128-163). */
+int afd_create(const afd_params *params, afd_ctx **out);
+int afd_destroy(afd_ctx *ctx);
+
+/* Host <-> device copies of the whole primitive array `pr` (n = nv * full cells). */
+int afd_set_prim(afd_ctx *ctx, const double *host, int64_t n);
+int afd_get_prim(afd_ctx *ctx, double *host, int64_t n);
+
+/* Nozzle profile; afd_nozzle_draw_count gives the uniform draws one fill consumes
+ * (3 per ghost cell with s > 0 when perturbed, in g -> j -> k order: core/nozzle.py:50-53). */
+int afd_set_nozzle(afd_ctx *ctx, const afd_nozzle *nozzle);
+int afd_nozzle_draw_count(afd_ctx *ctx, int64_t *ndraws);
+
+/* Ghost fills in the reference order: y/z (periodic, core/boundary.py:4-8, or outflow) ->
+ * +x outflow (core/boundary.py:10-11) -> -x nozzle (core/nozzle.py:27-92).
+ * draws: host uniforms in [0, 1), required when the nozzle is perturbed. */
+int afd_apply_bcs(afd_ctx *ctx, const double *draws, int64_t ndraws);
+int afd_apply_periodic_yz(afd_ctx *ctx);               /* core/boundary.py:4-8 */
+int afd_apply_outflow_right_x(afd_ctx *ctx);           /* core/boundary.py:10-11 */
+int afd_apply_nozzle_left_x(afd_ctx *ctx, const double *draws, int64_t ndraws); /* core/nozzle.py:27-92 */
+
+/* CFL speed of the current state: core/srhd_core.py:1267-1281 (max_char_speed). */
+int afd_max_char_speed(afd_ctx *ctx, double *amax);
+
+/* One SSPRK2/3 step of the current state, device resident:
+ * core/srhd_core.py:1286-1400, core/rmhd_core.py:1559-1677 (step_ssprk2/3). */
+int afd_step(afd_ctx *ctx, double dt);
+
+/* RHS of the current state, (nb + n_passive) x full cells, copied to host:
+ * core/srhd_core.py:468 (compute_rhs_muscl), :926 (ppm), :1166 (weno);
+ * core/rmhd_core.py:748 (compute_rhs_rmhd), :1279 (ppm), :1439 (weno). */
+int afd_compute_rhs(afd_ctx *ctx, double *rhs_host, int64_t n);
+
+/* core/source_terms.py:30-55 (apply_two_temperature) on the current state. */
+int afd_two_temperature(afd_ctx *ctx, double dt);
+
+/* Non-finite values in the current state (the driver's NaN check,
+ * solvers/srhd3d_mpi_muscl.py:337-343). */
+int afd_count_nonfinite(afd_ctx *ctx, int64_t *count);
+
+/* Pointwise conversions of n states (row-major n x nb) with the context's parameters:
+ * prim_to_cons (core/srhd_core.py:46, core/rmhd_core.py:137) and cons_to_prim
+ * (core/srhd_core.py:61, core/rmhd_core.py:248).  iters (optional, n): iterations used. */
+int afd_p2c_batch(afd_ctx *ctx, int64_t n, const double *prim, double *cons);
+int afd_c2p_batch(afd_ctx *ctx, int64_t n, const double *cons, double *prim, int32_t *iters);
+
+/* Counters and per-kernel HIP-event timings (no counterpart in the reference). */
+int afd_set_instrumentation(afd_ctx *ctx, int32_t stats, int32_t timing);
+int afd_get_stats(afd_ctx *ctx, afd_stats *out);
+int afd_reset_stats(afd_ctx *ctx);
+int afd_synchronize(afd_ctx *ctx);
+
+/* Learned con2prim guess (no counterpart in the reference): an MLP 5 -> 64 -> 64 -> 2 (SiLU)
+ * on scaled (D, |S|, tau, B^2, S.B) giving (log p, log W); BF16 weights, FP32 accumulation
+ * on MFMA.  Weights are the packed FP32 master copy (4674 values, layout in afd_mlp_layout). */
+int afd_mlp_param_count(int32_t *count);
+int afd_mlp_set_weights(afd_ctx *ctx, const float *w, int32_t count);
+int afd_mlp_get_weights(afd_ctx *ctx, float *w, int32_t count);
+
+const char *afd_last_error(void);
+const char *afd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

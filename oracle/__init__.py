@@ -93,6 +93,8 @@ def lib():
         L.orc_apply_nozzle_left_x.argtypes = [P, ctypes.POINTER(Nozzle), _D, _D, _I64]
         L.orc_apply_nozzle_left_x.restype = _I64
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_set_num_threads.argtypes = [ctypes.c_int]
+        L.orc_set_num_threads.restype = None
         _lib = L
     return _lib
 
@@ -285,3 +287,7 @@ def apply_two_temperature(P, pr, dt, tei_tau, thermo_offset):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def set_num_threads(n):
+    lib().orc_set_num_threads(int(n))

@@ -96,6 +96,7 @@ void orc_apply_two_temperature(const orc_params *P, double *pr, double dt, doubl
 
 /* number of OpenMP threads the whole-array operators use (1 when built without OpenMP) */
 int orc_num_threads(void);
+void orc_set_num_threads(int n);
 
 #ifdef __cplusplus
 }

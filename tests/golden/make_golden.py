@@ -60,43 +60,8 @@ def _save(name, **arrays):
 # ---------------------------------------------------------------------------
 # Seeded primitive states
 # ---------------------------------------------------------------------------
-def srhd_state(shape, seed, npass=0, jet=True):
-    rng = np.random.default_rng(seed)
-    nx, ny, nz = shape
-    pr = np.zeros((5 + npass, nx, ny, nz))
-    pr[0] = 1.0 + 0.3 * rng.uniform(-1, 1, shape)
-    pr[1] = 0.05 * rng.uniform(-1, 1, shape)
-    pr[2] = 0.05 * rng.uniform(-1, 1, shape)
-    pr[3] = 0.05 * rng.uniform(-1, 1, shape)
-    pr[4] = 0.01 * (1.0 + 0.5 * rng.uniform(-1, 1, shape))
-    if jet:
-        # a relativistic jet core (W ~ 7) crossing part of the block
-        sl = (slice(0, nx // 2 + 1), slice(ny // 3, 2 * ny // 3 + 1), slice(nz // 3, 2 * nz // 3 + 1))
-        pr[0][sl] = 0.01 * (1.0 + 0.2 * rng.uniform(-1, 1, pr[0][sl].shape))
-        pr[1][sl] = 0.99 - 0.01 * rng.uniform(0, 1, pr[1][sl].shape)
-        pr[4][sl] = 0.01 * (1.0 + 0.3 * rng.uniform(-1, 1, pr[4][sl].shape))
-    # a few quiescent cells (v = 0) exercise the first-iteration c2p exit
-    pr[1:4, 0, 0, :] = 0.0
-    for t in range(npass):
-        pr[5 + t] = rng.uniform(0.0, 1.0, shape)
-    return pr
-
-
-def rmhd_state(shape, seed, npass=0, jet=True):
-    rng = np.random.default_rng(seed)
-    nx, ny, nz = shape
-    pr = np.zeros((9 + npass, nx, ny, nz))
-    base = srhd_state(shape, seed + 1000, 0, jet)
-    pr[0:5] = base
-    pr[5] = 0.1 + 0.02 * rng.uniform(-1, 1, shape)
-    pr[6] = 0.05 * rng.uniform(-1, 1, shape)
-    pr[7] = 0.05 * rng.uniform(-1, 1, shape)
-    pr[8] = 1e-3 * rng.uniform(-1, 1, shape)
-    # one plane with Bn ~ 0 triggers the |Bn| < 1e-8 HLLE fallback of hll_d_flux_dir
-    pr[5, :, :, 1] = 0.0
-    for t in range(npass):
-        pr[9 + t] = rng.uniform(0.0, 1.0, shape)
-    return pr
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+from tests.golden_states import rmhd_state, srhd_state  # noqa: E402
 
 
 SRHD_BASE = dict(GAMMA=5.0 / 3.0, P_MAX=2.0, V_MAX=0.9999, N_PASSIVE=0, PASSIVE_OFFSET=5)

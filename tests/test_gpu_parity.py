@@ -1,0 +1,208 @@
+"""HIP path (libafd.so through the C-ABI) against the golden vectors and the oracle.
+
+The golden vectors hold the reference's own outputs; the oracle matches them
+bitwise (test_oracle_golden.py) and extends them to larger seeded states.  The
+GPU path contracts multiply-adds into FMAs in the face and con2prim kernels, so
+floating-point results agree to a relative tolerance, stated per test:
+  RHS (one evaluation)             max|diff| <= 1e-11 * max|rhs|
+  one SSPRK step                   max|diff| <= 1e-10 * max|q|  per variable
+  boundary / nozzle / source fills bitwise (no contraction in those kernels)
+  multi-step driver loops          max|diff| <= 1e-8  * max|q|  per variable
+"""
+import ast
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from tests import oracle_loop
+from tests.gpu_helpers import log, make_domain, rel_err
+
+pytestmark = pytest.mark.gpu
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return np.load(os.path.join(G, name + ".npz"))
+
+
+def per_var_rel(a, b):
+    return max(rel_err(a[v], b[v]) for v in range(b.shape[0]))
+
+
+# ---------------------------------------------------------------- RHS vs golden
+@pytest.mark.parametrize("phys", ["hydro", "rmhd"])
+def test_rhs_golden(phys):
+    f = load("srhd_rhs" if phys == "hydro" else "rmhd_rhs")
+    dx, dy, dz = f["dxyz"]
+    for c, name in enumerate(f["cases"]):
+        recon, lim, riem = str(name).split("/")
+        pr = f[f"pr_{c}"]
+        if phys == "hydro":
+            npass = int(f["npass"][c])
+            kw = dict(n_passive=npass)
+        else:
+            res = bool(f["resistive"][c])
+            nres = ({"muscl": 1, "ppm": 2, "weno": 0}[recon]) if res else 0
+            kw = dict(gamma=4.0 / 3.0, n_resist=nres, resistivity=1e-3 if res else 0.0)
+        log(f"[gpu] rhs {phys} {name} {kw}")
+        with make_domain(phys, pr.shape, 3, (dx, dy, dz), limiter=lim, recon=recon, riemann=riem, **kw) as d:
+            d.set_prim(pr)
+            rhs = d.compute_rhs()
+            if phys == "hydro":
+                assert abs(d.max_char_speed() - float(f[f"amax_{c}"])) <= 1e-14
+        err = rel_err(rhs, f[f"rhs_{c}"])
+        log(f"      rel err {err:.3e}")
+        assert err <= 1e-11, f"{phys} rhs {name}: {err}"
+
+
+# ---------------------------------------------------------------- steps vs golden
+@pytest.mark.parametrize("phys", ["hydro", "rmhd"])
+def test_step_golden(phys):
+    f = load("srhd_step" if phys == "hydro" else "rmhd_step")
+    dx, dy, dz = f["dxyz"]
+    dt = float(f["dt"])
+    for c, name in enumerate(f["cases"]):
+        parts = str(name).split("/")
+        recon, lim, riem, rk, npass = parts[0], parts[1], parts[2], int(parts[3]), int(parts[4])
+        pr = f[f"pr_{c}"]
+        kw = dict(n_passive=npass, rk_order=rk)
+        if phys == "rmhd":
+            res = parts[5] == "True"
+            nres = ({"muscl": 1, "ppm": 2, "weno": 0}[recon]) if res else 0
+            kw.update(gamma=4.0 / 3.0, n_resist=nres, resistivity=1e-3 if res else 0.0)
+        log(f"[gpu] step {phys} {name}")
+        with make_domain(phys, pr.shape, 3, (dx, dy, dz), limiter=lim, recon=recon, riemann=riem, **kw) as d:
+            d.set_prim(pr)
+            d.step(dt)
+            out = d.get_prim()
+        err = per_var_rel(out, f[f"out_{c}"])
+        log(f"      rel err {err:.3e}")
+        assert err <= 1e-10, f"{phys} step {name}: {err}"
+
+
+# ---------------------------------------------------------------- pointwise vs golden
+def test_c2p_p2c_golden():
+    f = load("srhd_pointwise")
+    with make_domain("hydro", (5, 12, 12, 12), 2, p_max=2.0, v_max=0.9999) as d:
+        log("[gpu] srhd p2c/c2p batch")
+        assert rel_err(d.p2c(f["prim"]), f["cons"]) <= 1e-15
+        out = d.c2p(f["cons_all"])
+        assert per_var_rel(out.T, f["c2p"].T) <= 1e-12
+    f = load("rmhd_pointwise")
+    with make_domain("rmhd", (9, 12, 12, 12), 2, gamma=4.0 / 3.0, p_max=1e4, v_max=0.9999) as d:
+        log("[gpu] rmhd p2c/c2p batch")
+        assert per_var_rel(d.p2c(f["prim"]).T, f["cons"].T) <= 1e-15
+        out = d.c2p(f["cons"])
+        err = np.abs(out - f["c2p"]) / (np.abs(f["c2p"]) + 1e-12)
+        log(f"      rmhd c2p max rel err {err.max():.3e}")
+        assert err.max() <= 1e-7
+        out = d.c2p(f["cons_bad"])
+        assert np.allclose(out, f["c2p_bad"], rtol=1e-7, atol=1e-12)
+
+
+# ---------------------------------------------------------------- boundaries vs golden
+def test_bc_nozzle_golden():
+    f = load("bc_nozzle")
+    NX, NY, NZ, ng = [int(x) for x in f["grid"]]
+    for c, name in enumerate(f["cases"]):
+        phys, prof, shear, pert, ntr, tt, binit = str(name).split("/")
+        ntr = int(ntr)
+        tt = tt == "True"
+        nvb = 9 if phys == "rmhd" else 5
+        s = dict(NX=NX, NY=NY, NZ=NZ, NG=ng, Lx=1.0, Ly=1.0, Lz=1.0, JET_RADIUS=0.3, SHEAR_THICK=float(shear),
+                 NOZZLE_PROFILE=prof, GAMMA_JET=7.0888121, ETA_RHO=0.01, P_EQ=0.01, RHO_AMB=1.0, VX_AMB=0.01,
+                 VY_AMB=-0.02, VZ_AMB=0.03, NOZZLE_PERTURB=pert, TURB_VAMP=0.02, TURB_PAMP=0.1, N_TRACERS=ntr,
+                 TRACER_OFFSET=nvb, TRACER_NOZZLE_VALUES=[1.0, 0.5][:ntr], TRACER_AMB_VALUES=[0.0, 0.25][:ntr],
+                 TWO_TEMPERATURE=tt, THERMO_OFFSET=nvb + ntr, TE_NOZZLE=0.7, TI_NOZZLE=0.9, PHYSICS=phys,
+                 B_INIT=binit, B0=0.5, JET_CENTER=[0.0, 0.45, 0.55])
+        pr = f[f"before_{c}"]
+        npass = pr.shape[0] - nvb
+        log(f"[gpu] bcs {name}")
+        with make_domain(phys, pr.shape, ng, (1.0 / NX, 1.0 / NY, 1.0 / NZ), n_passive=npass,
+                         two_temperature=tt, thermo_offset=nvb + ntr) as d:
+            d.set_prim(pr)
+            d.set_nozzle(s)
+            rng = np.random.default_rng(1234)
+            d.apply_periodic_yz()
+            d.apply_outflow_right_x()
+            d.apply_nozzle_left_x(rng)
+            d.apply_nozzle_left_x(rng)
+            got = d.get_prim()
+            assert np.array_equal(got, f[f"nozzle_{c}"]), f"bcs {name}"
+            d.two_temperature(0.01)
+            assert np.array_equal(d.get_prim(), f[f"twotemp_{c}"]), f"two-temperature {name}"
+
+
+# ---------------------------------------------------------------- driver loops vs golden
+def run_gpu_driver(s, pr0, nsteps, seed=1234):
+    from astrofd_amd.domain import Domain
+    d = Domain.from_settings(s)
+    try:
+        d.set_prim(pr0)
+        d.set_nozzle(s)
+        rng = np.random.default_rng(seed)
+        dx, dy, dz = s["Lx"] / s["NX"], s["Ly"] / s["NY"], s["Lz"] / s["NZ"]
+        d.apply_bcs(rng)
+        dts = []
+        for _ in range(nsteps):
+            amax = d.max_char_speed() if s["PHYSICS"] == "hydro" else 1.0
+            dt = s["CFL"] * min(dx, dy, dz) / max(amax, 1e-12)
+            d.apply_bcs(rng)
+            d.step(dt)
+            d.apply_bcs(rng)
+            d.two_temperature(dt)
+            dts.append(dt)
+        return d.get_prim(), np.array(dts)
+    finally:
+        d.close()
+
+
+@pytest.mark.parametrize("name", ["driver_srhd_10", "driver_rmhd_8", "driver_srhd_32"])
+def test_driver_golden(name):
+    path = os.path.join(G, name + ".npz")
+    if not os.path.exists(path):
+        pytest.skip(f"{name}.npz not generated")
+    f = np.load(path)
+    s = ast.literal_eval(str(f["settings"]))
+    s.setdefault("TRACER_NOZZLE_VALUES", [])
+    s.setdefault("TRACER_AMB_VALUES", [])
+    log(f"[gpu] driver {name}")
+    out, dts = run_gpu_driver(s, f["pr0"], len(f["dts"]))
+    e_dt = rel_err(dts, f["dts"])
+    err = per_var_rel(out, f["out"])
+    log(f"      dt rel err {e_dt:.3e}  state rel err {err:.3e}")
+    assert e_dt <= 1e-10
+    assert err <= 1e-8
+
+
+# ---------------------------------------------------------------- larger seeded states vs the oracle
+def seeded_state(phys, shape, seed):
+    from tests.golden_states import rmhd_state, srhd_state
+    return (rmhd_state if phys == "rmhd" else srhd_state)(shape, seed)
+
+
+@pytest.mark.parametrize("phys,recon,lim,riem,rk", [
+    ("hydro", "muscl", "vanleer", "hllc", 3),
+    ("hydro", "weno", "mc", "hlle", 2),
+    ("rmhd", "weno", "mc", "hlld_full", 3),
+    ("rmhd", "ppm", "mc", "hlld", 3),
+    ("rmhd", "muscl", "minmod", "hlld", 2),
+])
+def test_step_vs_oracle_24(phys, recon, lim, riem, rk):
+    shape = ((9 if phys == "rmhd" else 5), 30, 28, 26)
+    pr = seeded_state(phys, shape[1:], 77)
+    dx = dy = dz = 1.0 / 24
+    kw = dict(gamma=4.0 / 3.0) if phys == "rmhd" else {}
+    P = O.make_params(phys, shape[1:], ng=3, gamma=kw.get("gamma"), p_max=2.0, v_max=0.9999, limiter=lim,
+                      recon=recon, riemann=riem, rk_order=rk)
+    ref = O.step(P, pr, dx, dy, dz, 0.4 * dx)
+    log(f"[gpu] step vs oracle {phys} {recon} {lim} {riem} rk{rk} {shape}")
+    with make_domain(phys, shape, 3, (dx, dy, dz), limiter=lim, recon=recon, riemann=riem, rk_order=rk, **kw) as d:
+        d.set_prim(pr)
+        d.step(0.4 * dx)
+        out = d.get_prim()
+    err = per_var_rel(out, ref)
+    log(f"      rel err {err:.3e}")
+    assert err <= 1e-10
