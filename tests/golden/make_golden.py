@@ -61,7 +61,7 @@ def _save(name, **arrays):
 # Seeded primitive states
 # ---------------------------------------------------------------------------
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
-from tests.golden_states import rmhd_state, srhd_state  # noqa: E402
+from tests.golden_states import rmhd_state, smooth_state, srhd_state  # noqa: E402
 
 
 SRHD_BASE = dict(GAMMA=5.0 / 3.0, P_MAX=2.0, V_MAX=0.9999, N_PASSIVE=0, PASSIVE_OFFSET=5)
@@ -327,9 +327,12 @@ def gen_steps():
     for c, (recon, lim, riem, rk, npass, res) in enumerate(RMHD_STEP_CASES):
         rmhd_configure(RECON=recon, LIMITER=lim, RIEMANN=riem, N_PASSIVE=npass, PASSIVE_OFFSET=9,
                        RESISTIVE_ENABLED=res, RESISTIVITY=1e-3 if res else 0.0)
-        pr = rmhd_state(shape, 400 + c, npass=npass)
+        # smooth states: rough ones make the reference RMHD step diverge within one step
+        pr = smooth_state(shape, 400 + c, "rmhd", vamp=0.2, npass=npass)
         t0 = time.time()
         out = (rmhd_core.step_ssprk3 if rk == 3 else rmhd_core.step_ssprk2)(pr, dx, dy, dz, dt)
+        print(f"[golden]   out rho [{out[0].min():.3e}, {out[0].max():.3e}] p max {out[4].max():.3e} "
+              f"|B| max {np.abs(out[5:8]).max():.3e}", flush=True)
         arrs[f"pr_{c}"] = pr; arrs[f"out_{c}"] = out
         print(f"[golden] rmhd step {recon}/{lim}/{riem}/rk{rk}/np{npass}/res{res} {time.time()-t0:.1f}s",
               flush=True)
@@ -451,16 +454,19 @@ def init_block(s, noise_seed):
 def gen_driver_small():
     # RMHD WENO5 + hlld_full + SSPRK3, NG = 3, random nozzle perturbation, toroidal nozzle field,
     # two-temperature passives, resistivity: exercises the whole loop order on a tiny grid.
+    # a mildly relativistic jet (v = 0.3): the reference RMHD step diverges for W ~ 7 jets
     s = driver_settings(NX=8, NY=8, NZ=8, NG=3, PHYSICS="rmhd", GAMMA=4.0 / 3.0, RECON="weno",
-                        RIEMANN="hlld_full", RK_ORDER=3, B_INIT="toroidal", B0=0.5, NOZZLE_PERTURB="random",
-                        TURB_PAMP=0.05, JET_RADIUS=0.3, TWO_TEMPERATURE=True, TEI_TAU=0.1, TE_AMB=0.01,
+                        RIEMANN="hlld_full", RK_ORDER=3, B_INIT="toroidal", B0=0.05, NOZZLE_PERTURB="random",
+                        TURB_PAMP=0.05, JET_RADIUS=0.3, GAMMA_JET=1.0482848, ETA_RHO=0.5, SHEAR_THICK=0.05,
+                        TWO_TEMPERATURE=True, TEI_TAU=0.1, TE_AMB=0.01,
                         TI_AMB=0.01, TE_NOZZLE=0.5, TI_NOZZLE=0.2, N_PASSIVE=2, PASSIVE_OFFSET=9,
                         THERMO_OFFSET=9, TRACER_OFFSET=9, RESISTIVE_ENABLED=True, RESISTIVITY=1e-3)
     pr0 = init_block(s, 12345)
     pr0[5] = 0.1  # axial field everywhere (as the RMHD benchmark's B_AMB)
     t0 = time.time()
     out, dts = run_driver(s, pr0.copy(), 3, np.random.default_rng(1234))
-    print(f"[golden] rmhd driver {time.time()-t0:.1f}s", flush=True)
+    print(f"[golden] rmhd driver {time.time()-t0:.1f}s: rho [{out[0].min():.3e}, {out[0].max():.3e}] "
+          f"p max {out[4].max():.3e} |B| max {np.abs(out[5:8]).max():.3e}", flush=True)
     _save("driver_rmhd_8", pr0=pr0, out=out, dts=dts, settings=np.array(repr(s)))
 
     s = driver_settings(NX=10, NY=9, NZ=8, NG=2, RECON="muscl", LIMITER="mc", RIEMANN="hllc", RK_ORDER=3,

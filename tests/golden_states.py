@@ -37,3 +37,39 @@ def rmhd_state(shape, seed, npass=0, jet=True):
     for t in range(npass):
         pr[9 + t] = rng.uniform(0.0, 1.0, shape)
     return pr
+
+
+def smooth_state(shape, seed, physics="hydro", vamp=0.2, npass=0):
+    """Well-conditioned state: rho, p, v (and B, psi) as a few random Fourier modes around
+    rho = 1, p = 0.05, |v| <= vamp, B ~ 0.1 (RMHD).  Used where a step is compared across
+    implementations: rough states make the reference scheme diverge chaotically within one step."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = shape
+    x = np.arange(nx)[:, None, None] / nx
+    y = np.arange(ny)[None, :, None] / ny
+    z = np.arange(nz)[None, None, :] / nz
+
+    def field(amp):
+        f = np.zeros(shape)
+        for _ in range(4):
+            k = rng.integers(1, 3, 3)
+            ph = rng.uniform(0, 2 * np.pi, 3)
+            f += np.sin(2 * np.pi * k[0] * x + ph[0]) * np.sin(2 * np.pi * k[1] * y + ph[1]) * \
+                np.sin(2 * np.pi * k[2] * z + ph[2])
+        return amp * f / 4.0
+
+    nb = 9 if physics == "rmhd" else 5
+    pr = np.zeros((nb + npass,) + tuple(shape))
+    pr[0] = 1.0 + field(0.3)
+    pr[1] = field(vamp)
+    pr[2] = field(vamp)
+    pr[3] = field(vamp)
+    pr[4] = 0.05 * (1.0 + field(0.5))
+    if physics == "rmhd":
+        pr[5] = 0.1 + field(0.05)
+        pr[6] = field(0.05)
+        pr[7] = field(0.05)
+        pr[8] = field(1e-3)
+    for t in range(npass):
+        pr[nb + t] = 0.5 + field(0.4)
+    return pr
