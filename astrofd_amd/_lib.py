@@ -95,6 +95,11 @@ SIGNATURES = {
     "afd_mlp_param_count": [ctypes.POINTER(_i32)],
     "afd_mlp_set_weights": [_VP, ctypes.POINTER(_f32), _i32],
     "afd_mlp_get_weights": [_VP, ctypes.POINTER(_f32), _i32],
+    "afd_mlp_dataset_from_prims": [_VP, _i64, _D, ctypes.POINTER(_f32), ctypes.POINTER(_f32), ctypes.POINTER(_i32)],
+    "afd_mlp_set_dataset": [_VP, _i64, ctypes.POINTER(_f32), ctypes.POINTER(_f32)],
+    "afd_mlp_train": [_VP, _i64, _i32, _f32, _D],
+    "afd_mlp_forward": [_VP, _i64, ctypes.POINTER(_f32), ctypes.POINTER(_f32)],
+    "afd_mlp_loss_grad": [_VP, _i64, ctypes.POINTER(_f32), ctypes.POINTER(_f32), ctypes.POINTER(_f32), _D],
 }
 STRING_FUNCS = ["afd_last_error", "afd_version"]
 

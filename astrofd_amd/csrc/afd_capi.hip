@@ -675,8 +675,10 @@ static int batch(afd_ctx *c, int64_t n, const double *in, double *out, int32_t *
     if (e == hipSuccess && iters) e = hipMalloc(&dit, (size_t)n * 4);
     if (e == hipSuccess) e = hipMemcpyAsync(din, in, bytes, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
-        BatchArgs a{din, dout, dit, (long)n, c->p.physics, c->p.gamma, c->p.gamma / (c->p.gamma - 1.0), c->p.p_max,
-                    c->p.v_max};
+        BatchArgs a{din,           dout,        dit,        c2p && c->stats_on ? c->stats_dev : nullptr,
+                    (long)n,       c->p.physics, c->p.gamma, c->p.gamma / (c->p.gamma - 1.0),
+                    c->p.p_max,    c->p.v_max};
+        if (c2p && c->stats_on) c->acc.c2p_cells += (uint64_t)n;
         if (c2p) {
             C2PBatchExtra x{c->p.c2p_guess == AFD_C2P_LEARNED && c->mlp.loaded ? c->mlp.w : nullptr};
             launch_c2p_batch(a, x, c->stream);
@@ -758,7 +760,7 @@ extern "C" int afd_mlp_set_weights(afd_ctx *c, const float *w, int32_t count) {
     HIPCHK(hipSetDevice(c->p.device));
     HIPCHK(mlp_upload(c->mlp, w, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    c->mlp.loaded = true;
+    c->mlp.loaded = true;  // the host copy may be released: the upload has completed
     return AFD_OK;
 }
 
@@ -769,5 +771,152 @@ extern "C" int afd_mlp_get_weights(afd_ctx *c, float *w, int32_t count) {
     HIPCHK(hipSetDevice(c->p.device));
     HIPCHK(hipMemcpyAsync(w, c->mlp.master, MLP_NPARAM * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    return AFD_OK;
+}
+
+// ------------------------------------------------------------------ learned guess: data, training, checks
+static int need_mlp(afd_ctx *c) {
+    if (!c) return fail(AFD_ERR_INVALID, "null context");
+    if (c->nb != 9) return fail(AFD_ERR_STATE, "the learned guess is RMHD only");
+    return AFD_OK;
+}
+
+template <typename T> static int tmp_alloc(T **p, size_t bytes) {
+    hipError_t e = hipMalloc((void **)p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(AFD_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    return AFD_OK;
+}
+
+extern "C" int afd_mlp_dataset_from_prims(afd_ctx *c, int64_t n, const double *prim, float *X, float *Y,
+                                          int32_t *valid) {
+    if (int rc = need_mlp(c)) return rc;
+    if (n <= 0 || !prim || !X || !Y || !valid) return fail(AFD_ERR_INVALID, "afd_mlp_dataset_from_prims: bad argument");
+    HIPCHK(hipSetDevice(c->p.device));
+    const int64_t chunk = 1 << 22;
+    double *dp = nullptr;
+    float *dX = nullptr, *dY = nullptr;
+    int *dv = nullptr;
+    int64_t m = n < chunk ? n : chunk;
+    int rc = tmp_alloc(&dp, (size_t)m * 72);
+    if (!rc) rc = tmp_alloc(&dX, (size_t)m * 20);
+    if (!rc) rc = tmp_alloc(&dY, (size_t)m * 8);
+    if (!rc) rc = tmp_alloc(&dv, (size_t)m * 4);
+    hipError_t e = hipSuccess;
+    for (int64_t off = 0; !rc && e == hipSuccess && off < n; off += chunk) {
+        int64_t k = (n - off) < chunk ? (n - off) : chunk;
+        e = hipMemcpyAsync(dp, prim + 9 * off, (size_t)k * 72, hipMemcpyHostToDevice, c->stream);
+        MlpDataArgs a{dp, dX, dY, dv, (long)k, c->p.gamma, c->p.gamma / (c->p.gamma - 1.0), c->p.p_max, c->p.v_max};
+        if (e == hipSuccess) e = mlp_prepare_dataset(a, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(X + 5 * off, dX, (size_t)k * 20, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(Y + 2 * off, dY, (size_t)k * 8, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(valid + off, dv, (size_t)k * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    (void)hipFree(dp);
+    (void)hipFree(dX);
+    (void)hipFree(dY);
+    (void)hipFree(dv);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(AFD_ERR_HIP, std::string("dataset: ") + hipGetErrorString(e));
+    return AFD_OK;
+}
+
+static void forget_alloc(afd_ctx *c, void *ptr) {
+    for (auto it = c->allocs.begin(); it != c->allocs.end(); ++it)
+        if (*it == ptr) {
+            c->allocs.erase(it);
+            break;
+        }
+    (void)hipFree(ptr);
+}
+
+extern "C" int afd_mlp_set_dataset(afd_ctx *c, int64_t n, const float *X, const float *Y) {
+    if (int rc = need_mlp(c)) return rc;
+    if (n <= 0 || !X || !Y) return fail(AFD_ERR_INVALID, "afd_mlp_set_dataset: bad argument");
+    HIPCHK(hipSetDevice(c->p.device));
+    MlpDevice &m = c->mlp;
+    if (n > m.cap_data) {
+        if (m.X) forget_alloc(c, m.X);
+        if (m.Y) forget_alloc(c, m.Y);
+        m.X = m.Y = nullptr;
+        m.cap_data = 0;
+        ALLOC(c, m.X, (size_t)n * 20);
+        ALLOC(c, m.Y, (size_t)n * 8);
+        m.cap_data = n;
+    }
+    HIPCHK(hipMemcpyAsync(m.X, X, (size_t)n * 20, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(m.Y, Y, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    m.n_data = n;
+    return AFD_OK;
+}
+
+extern "C" int afd_mlp_train(afd_ctx *c, int64_t steps, int32_t batch, float lr, double *loss_history) {
+    if (int rc = need_mlp(c)) return rc;
+    MlpDevice &m = c->mlp;
+    if (!m.loaded) return fail(AFD_ERR_STATE, "initial weights not set (afd_mlp_set_weights)");
+    if (m.n_data <= 0) return fail(AFD_ERR_STATE, "no dataset (afd_mlp_set_dataset)");
+    if (batch <= 0 || batch > m.n_data) return fail(AFD_ERR_INVALID, "batch must be in [1, dataset size]");
+    if (steps < 0 || steps > 65536) return fail(AFD_ERR_INVALID, "steps must be in [0, 65536] per call");
+    HIPCHK(hipSetDevice(c->p.device));
+    const int64_t nbat = m.n_data / batch;
+    for (int64_t st = 0; st < steps; ++st) {
+        int64_t off = (st % nbat) * batch;
+        HIPCHK(mlp_loss_grad(m.w, m.X + 5 * off, m.Y + 2 * off, batch, m.grad, m.loss + st, c->stream));
+        HIPCHK(mlp_adam(m.master, m.adam_m, m.adam_v, m.grad, ++m.adam_t, lr, c->stream));
+        HIPCHK(mlp_pack(m.master, m.w, c->stream));
+    }
+    if (loss_history && steps > 0)
+        HIPCHK(hipMemcpyAsync(loss_history, m.loss, (size_t)steps * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AFD_OK;
+}
+
+extern "C" int afd_mlp_forward(afd_ctx *c, int64_t n, const float *X, float *Y) {
+    if (int rc = need_mlp(c)) return rc;
+    if (!c->mlp.loaded) return fail(AFD_ERR_STATE, "no weights loaded");
+    if (n <= 0 || !X || !Y) return fail(AFD_ERR_INVALID, "afd_mlp_forward: bad argument");
+    HIPCHK(hipSetDevice(c->p.device));
+    float *dX = nullptr, *dY = nullptr;
+    int rc = tmp_alloc(&dX, (size_t)n * 20);
+    if (!rc) rc = tmp_alloc(&dY, (size_t)n * 8);
+    hipError_t e = hipSuccess;
+    if (!rc) {
+        e = hipMemcpyAsync(dX, X, (size_t)n * 20, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = mlp_forward(c->mlp.w, dX, n, dY, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(Y, dY, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    (void)hipFree(dX);
+    (void)hipFree(dY);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(AFD_ERR_HIP, std::string("mlp forward: ") + hipGetErrorString(e));
+    return AFD_OK;
+}
+
+extern "C" int afd_mlp_loss_grad(afd_ctx *c, int64_t n, const float *X, const float *Y, float *grad, double *loss) {
+    if (int rc = need_mlp(c)) return rc;
+    if (!c->mlp.loaded) return fail(AFD_ERR_STATE, "no weights loaded");
+    if (n <= 0 || !X || !Y || !grad || !loss) return fail(AFD_ERR_INVALID, "afd_mlp_loss_grad: bad argument");
+    HIPCHK(hipSetDevice(c->p.device));
+    float *dX = nullptr, *dY = nullptr;
+    int rc = tmp_alloc(&dX, (size_t)n * 20);
+    if (!rc) rc = tmp_alloc(&dY, (size_t)n * 8);
+    hipError_t e = hipSuccess;
+    if (!rc) {
+        e = hipMemcpyAsync(dX, X, (size_t)n * 20, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(dY, Y, (size_t)n * 8, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = mlp_loss_grad(c->mlp.w, dX, dY, n, c->mlp.grad, c->mlp.loss, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(grad, c->mlp.grad, MLP_NPARAM * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(loss, c->mlp.loss, 8, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    (void)hipFree(dX);
+    (void)hipFree(dY);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(AFD_ERR_HIP, std::string("mlp loss/grad: ") + hipGetErrorString(e));
     return AFD_OK;
 }

@@ -52,21 +52,23 @@ __global__ __launch_bounds__(BLOCK) void k_passive_rhs(PassiveArgs a) {
 
 __global__ __launch_bounds__(BLOCK) void k_c2p_batch(BatchArgs a) {
     long m = (long)blockIdx.x * BLOCK + threadIdx.x;
-    if (m >= a.n) return;
     C2PStats st = {0, 0, 0};
-    if (a.physics == 1) {
-        double U[9], q[9];
-        for (int v = 0; v < 9; ++v) U[v] = a.in[9 * m + v];
-        RmhdC2PParams cp = {a.gamma, a.gfac, a.p_max, a.v_max};
-        rmhd_c2p(cp, U, q, st);
-        for (int v = 0; v < 9; ++v) a.out[9 * m + v] = q[v];
-    } else {
-        double U[5], q[5];
-        for (int v = 0; v < 5; ++v) U[v] = a.in[5 * m + v];
-        hydro_c2p(a.gamma, a.gfac, a.p_max, a.v_max, U, q, st);
-        for (int v = 0; v < 5; ++v) a.out[5 * m + v] = q[v];
+    if (m < a.n) {
+        if (a.physics == 1) {
+            double U[9], q[9];
+            for (int v = 0; v < 9; ++v) U[v] = a.in[9 * m + v];
+            RmhdC2PParams cp = {a.gamma, a.gfac, a.p_max, a.v_max};
+            rmhd_c2p(cp, U, q, st);
+            for (int v = 0; v < 9; ++v) a.out[9 * m + v] = q[v];
+        } else {
+            double U[5], q[5];
+            for (int v = 0; v < 5; ++v) U[v] = a.in[5 * m + v];
+            hydro_c2p(a.gamma, a.gfac, a.p_max, a.v_max, U, q, st);
+            for (int v = 0; v < 5; ++v) a.out[5 * m + v] = q[v];
+        }
+        if (a.iters) a.iters[m] = st.hydro_iters + st.newton_iters;
     }
-    if (a.iters) a.iters[m] = st.hydro_iters + st.newton_iters;
+    if (a.stats) wave_add_stats(a.stats, st);  // reached by every lane
 }
 
 __global__ __launch_bounds__(BLOCK) void k_p2c_batch(BatchArgs a) {
@@ -273,7 +275,12 @@ template <int NB, typename F> static void upd(const UpdLaunch &l, bool rhs_only,
     memcpy((void *)&a, (const void *)&l.a, sizeof(a));
     dim3 grid = grid1d(l.a.g.ncell);
     if (rhs_only) hipLaunchKernelGGL((k_rhs<NB, F>), grid, dim3(BLOCK), 0, s, a);
-    else hipLaunchKernelGGL((k_update<NB, F>), grid, dim3(BLOCK), 0, s, a);
+    else if constexpr (NB == 9) {
+        if (l.a.mlp) hipLaunchKernelGGL((k_update<NB, F, true>), grid, dim3(BLOCK), 0, s, a);
+        else hipLaunchKernelGGL((k_update<NB, F, false>), grid, dim3(BLOCK), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((k_update<NB, F, false>), grid, dim3(BLOCK), 0, s, a);
+    }
 }
 
 hipError_t launch_update(const UpdLaunch &l, bool rhs_only, hipStream_t s) {

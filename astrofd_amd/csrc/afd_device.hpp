@@ -579,8 +579,9 @@ AFD_D bool rmhd_newton(const RmhdC2PParams &c, double &Z, double D, double tau, 
     return false;
 }
 
-// Recover the primitives from Z (core/rmhd_core.py:320-341); hydro guess q0 replaces invalid results
-AFD_D void rmhd_recover(const RmhdC2PParams &c, double Z, const double *U, double B2, double SB, const double *q0,
+// Recover the primitives from Z (core/rmhd_core.py:320-341); the hydro guess q0 replaces invalid
+// results.  With q0 == null an invalid recovery is reported (false) and q is left unset.
+AFD_D bool rmhd_recover(const RmhdC2PParams &c, double Z, const double *U, double B2, double SB, const double *q0,
                         double *q, C2PStats &st) {
     double Sx = U[1], Sy = U[2], Sz = U[3], D = U[0], Bx = U[5], By = U[6], Bz = U[7];
     double ZpB = Z + B2;
@@ -601,6 +602,7 @@ AFD_D void rmhd_recover(const RmhdC2PParams &c, double Z, const double *U, doubl
     double h = Z / pymax(rho * W * W, SMALL);
     double p = (h - 1.0) * rho * (c.gamma - 1.0) / c.gamma;
     if (!isfinite(rho) || !isfinite(p) || rho <= 0.0 || p <= 0.0) {
+        if (!q0) return false;
         st.flags |= 4;
         rho = q0[0];
         vx = q0[1];
@@ -618,6 +620,7 @@ AFD_D void rmhd_recover(const RmhdC2PParams &c, double Z, const double *U, doubl
     q[7] = Bz;
     q[8] = U[8];
     rmhd_floor(c.p_max, c.v_max, q);
+    return true;
 }
 
 // core/rmhd_core.py:248-341 (hydro guess, FD-Newton from Z0 then Z1, bisection, recovery)
@@ -675,6 +678,22 @@ AFD_D void rmhd_c2p(const RmhdC2PParams &c, const double *U, double *q, C2PStats
         }
     }
     rmhd_recover(c, Z, U, B2, SB, q0, q, st);
+}
+
+// Learned-guess con2prim: FD-Newton (the reference's iteration) from the network's Z0, judged by a
+// residual |f(Z)| < 1e-12 max(1, tau), a hundred times tighter than the reference's stopping test, and
+// by a valid recovery.  Returns false when the judge rejects; the caller then runs rmhd_c2p.
+AFD_D bool rmhd_c2p_from_guess(const RmhdC2PParams &c, const double *U, double Z0, double *q, C2PStats &st) {
+    if (!(Z0 > SMALL) || !isfinite(Z0)) return false;
+    double D = U[0], Sx = U[1], Sy = U[2], Sz = U[3], tau = U[4], Bx = U[5], By = U[6], Bz = U[7];
+    double B2 = Bx * Bx + By * By + Bz * Bz;
+    double SB = Sx * Bx + Sy * By + Sz * Bz;
+    double S2 = Sx * Sx + Sy * Sy + Sz * Sz;
+    double Z = Z0;
+    if (!rmhd_newton(c, Z, D, tau, B2, SB, S2, 1e-12 * pymax(1.0, tau), 12, st)) return false;
+    if (!rmhd_recover(c, Z, U, B2, SB, nullptr, q, st)) return false;
+    st.flags |= 8;
+    return true;
 }
 
 // FP64 p2c of a cell state (stage-1 U0): core/srhd_core.py:46-59, core/rmhd_core.py:137-168

@@ -12,6 +12,7 @@
 // (the left face of cell c).  The z kernel runs first and stores those speeds.
 #pragma once
 #include "afd_device.hpp"
+#include "afd_mlp_device.hpp"
 
 #include <cstring>
 
@@ -385,20 +386,25 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(UpdArgs<F> a) {
     for (int t = 0; t < a.np; ++t) a.Pout[(NB + t) * g.ncell + c] = a.rhs_pass ? a.rhs_pass[t * g.ncell + c] : 0.0;
 }
 
-template <int NB, typename F>
+// LEARNED: the RMHD con2prim starts from the network's guess (a.mlp); a separate instantiation
+// keeps the network's registers out of the plain kernel.
+template <int NB, typename F, bool LEARNED>
 __global__ __launch_bounds__(BLOCK) void k_update(UpdArgs<F> a) {
     const Grid &g = a.g;
-    long c = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const long c = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const bool live = c < g.ncell;
     C2PStats st = {0, 0, 0};
-    if (c < g.ncell) {
+    double U[NB], q[NB];
+#pragma unroll
+    for (int v = 0; v < NB; ++v) U[v] = 0.0;
+    if (live) {
         double rhs[NB];
         cell_rhs<NB, F>(a, c, in_range(g, c), rhs);
-        double U[NB], q[NB];
         if (a.stage == 0) {
             double q0[NB];
 #pragma unroll
             for (int v = 0; v < NB; ++v) q0[v] = a.P0[v * g.ncell + c];
-            if (NB == 9) rmhd_p2c_cell(a.gfac, a.p_max, a.v_max, q0, U);
+            if constexpr (NB == 9) rmhd_p2c_cell(a.gfac, a.p_max, a.v_max, q0, U);
             else srhd_p2c_cell(a.gfac, q0, U);
 #pragma unroll
             for (int v = 0; v < NB; ++v) a.U0[v * g.ncell + c] = U[v];
@@ -417,12 +423,21 @@ __global__ __launch_bounds__(BLOCK) void k_update(UpdArgs<F> a) {
             U[v] = ssprk_combine(a.stage, a.nst, U[v], uc, r);
             a.Uc[v * g.ncell + c] = U[v];
         }
-        if (NB == 9) {
-            RmhdC2PParams cp = {a.gamma, a.gfac, a.p_max, a.v_max};
-            rmhd_c2p(cp, U, q, st);
-        } else {
-            hydro_c2p(a.gamma, a.gfac, a.p_max, a.v_max, U, q, st);
+    }
+    if constexpr (NB == 9) {
+        RmhdC2PParams cp = {a.gamma, a.gfac, a.p_max, a.v_max};
+        bool done = false;
+        if constexpr (LEARNED) {  // every lane of the wave runs the network
+            float x[5], y[2];
+            bool fok = mlp_features(U, x) && live;
+            mlp_infer_wave(mlp_view(a.mlp), x, y);
+            if (fok) done = rmhd_c2p_from_guess(cp, U, mlp_guess_Z(U[0], a.gfac, y), q, st);
         }
+        if (live && !done) rmhd_c2p(cp, U, q, st);
+    } else {
+        if (live) hydro_c2p(a.gamma, a.gfac, a.p_max, a.v_max, U, q, st);
+    }
+    if (live) {
 #pragma unroll
         for (int v = 0; v < NB; ++v) a.Pout[v * g.ncell + c] = q[v];
         for (int t = 0; t < a.np; ++t) {
@@ -444,6 +459,7 @@ struct BatchArgs {
     const double *in;
     double *out;
     int *iters;  // per state: hydro iterations + Newton evaluations (or null)
+    unsigned long long *stats;  // solver counters (or null), as in the stage update
     long n;
     int physics;
     double gamma, gfac, p_max, v_max;

@@ -1,0 +1,192 @@
+// afd_mlp_device.hpp -- wave-level BF16 MFMA multilayer perceptron (5 -> 64 -> 64 -> 2, SiLU).
+//
+// One wavefront evaluates 64 samples as two tiles of 32 with v_mfma_f32_32x32x16_bf16.
+// Activations are kept transposed (features x samples) so that samples run along the
+// lanes: a layer is  Out^T (rows x 32 samples) = W (rows x K) * In^T (K x 32 samples)
+// with the weights as the A operand (pre-packed in lane order, BF16) and the activations
+// as the B operand.  MFMA operand layouts (32x32x16, 64 lanes, l = lane):
+//   A (32 x 16):  lane l holds A[l % 32][8 (l / 32) + e],  e = 0..7
+//   B (16 x 32):  lane l holds B[8 (l / 32) + e][l % 32]
+//   D (32 x 32):  lane l holds D[8 (i / 4) + 4 (l / 32) + i % 4][l % 32],  i = 0..15
+// A D-layout result becomes the next layer's B operand after one exchange of four values
+// between lanes l and l ^ 32 per 16-row K block.  Accumulation is FP32.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "afd_device.hpp"
+
+namespace afd {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// Packed device weights: NBLK blocks of 64 lanes x 8 BF16 (A operands), then FP32 biases.
+enum MlpBlock {
+    BLK_W1 = 0,    // W1 [64 x 16]: m-tile 0..1, k-block 0 (inputs 5..15 zero)
+    BLK_W2 = 2,    // W2 [64 x 64]: m-tile 0..1 x k-block 0..3  (index 2 + 4 mt + kb)
+    BLK_W3 = 10,   // W3 [32 x 64]: k-block 0..3 (rows 2..31 zero)
+    BLK_W3T = 14,  // W3^T [64 x 16]: m-tile 0..1 (columns 2..15 zero)
+    BLK_W2T = 16,  // W2^T [64 x 64]: m-tile 0..1 x k-block 0..3
+    MLP_NBLK = 24
+};
+constexpr int MLP_BIAS_FLOATS = 64 + 64 + 2;
+
+struct MlpDevW {
+    const bf16x8 *blk;  // MLP_NBLK * 64 vectors
+    const float *bias;  // b1[64], b2[64], b3[2]
+};
+
+AFD_D MlpDevW mlp_view(const void *w) {
+    MlpDevW v;
+    v.blk = (const bf16x8 *)w;
+    v.bias = (const float *)((const char *)w + (size_t)MLP_NBLK * 64 * 16);
+    return v;
+}
+
+AFD_D floatx16 mfma_bf16(bf16x8 a, bf16x8 b, floatx16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+AFD_D floatx16 zero16() {
+    floatx16 z;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) z[i] = 0.0f;
+    return z;
+}
+
+AFD_D int drow(int i, int lane) { return 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3); }
+
+AFD_D float silu(float x) { return x * __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
+// d/dx silu(x) = s (1 + x (1 - s)), s = sigmoid(x)
+AFD_D float dsilu(float x) {
+    float s = __builtin_amdgcn_rcpf(1.0f + __expf(-x));
+    return s * (1.0f + x * (1.0f - s));
+}
+
+AFD_D bf16x8 pack8(float a0, float a1, float a2, float a3, float a4, float a5, float a6, float a7) {
+    bf16x8 r;
+    r[0] = (__bf16)a0; r[1] = (__bf16)a1; r[2] = (__bf16)a2; r[3] = (__bf16)a3;
+    r[4] = (__bf16)a4; r[5] = (__bf16)a5; r[6] = (__bf16)a6; r[7] = (__bf16)a7;
+    return r;
+}
+
+// D layout of a 64-row activation (two 32-row m-tiles) -> B operands of its four 16-row K blocks.
+AFD_D void d_to_b(const floatx16 (&d)[2], int lane, bf16x8 (&bop)[4]) {
+    const bool upper = lane >= 32;
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+            float lo[4], hi[4], rv[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                lo[r] = d[mt][8 * kk + r];      // rows 16kk + 4 (l/32) + r
+                hi[r] = d[mt][8 * kk + 4 + r];  // rows 16kk + 8 + 4 (l/32) + r
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) rv[r] = __shfl_xor(upper ? lo[r] : hi[r], 32);
+            bop[2 * mt + kk] = upper ? pack8(rv[0], rv[1], rv[2], rv[3], hi[0], hi[1], hi[2], hi[3])
+                                     : pack8(lo[0], lo[1], lo[2], lo[3], rv[0], rv[1], rv[2], rv[3]);
+        }
+    }
+}
+
+// Forward pass of one 32-sample tile.  xb: B operand of the inputs (lanes < 32 hold the five
+// features of sample l, lanes >= 32 zeros).  Returns pre-activations z1, z2 (bias included),
+// the BF16 B operands of h1, h2 and the output tile o (rows 0, 1 = outputs at lanes < 32).
+struct MlpTile {
+    floatx16 z1[2], z2[2], o;
+    bf16x8 h1b[4], h2b[4];
+};
+
+AFD_D void mlp_tile_forward(const MlpDevW &w, int lane, bf16x8 xb, MlpTile &t) {
+    floatx16 h[2];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+        t.z1[mt] = mfma_bf16(w.blk[(BLK_W1 + mt) * 64 + lane], xb, zero16());
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            t.z1[mt][i] += w.bias[32 * mt + drow(i, lane)];
+            h[mt][i] = silu(t.z1[mt][i]);
+        }
+    }
+    d_to_b(h, lane, t.h1b);
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+        floatx16 acc = zero16();
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) acc = mfma_bf16(w.blk[(BLK_W2 + 4 * mt + kb) * 64 + lane], t.h1b[kb], acc);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            acc[i] += w.bias[64 + 32 * mt + drow(i, lane)];
+            h[mt][i] = silu(acc[i]);
+        }
+        t.z2[mt] = acc;
+    }
+    d_to_b(h, lane, t.h2b);
+    floatx16 o = zero16();
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) o = mfma_bf16(w.blk[(BLK_W3 + kb) * 64 + lane], t.h2b[kb], o);
+    o[0] += w.bias[128];
+    o[1] += w.bias[129];
+    t.o = o;
+}
+
+AFD_D bf16x8 input_operand(const float x[5], bool upper) {
+    return upper ? pack8(0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f) : pack8(x[0], x[1], x[2], x[3], x[4], 0.f, 0.f, 0.f);
+}
+
+// Whole-wave inference: lane l supplies the features of its sample, gets its two outputs.
+// Must be reached by all 64 lanes of the wave.
+AFD_D void mlp_infer_wave(const MlpDevW &w, const float x[5], float y[2]) {
+    const int lane = threadIdx.x & 63;
+    const bool upper = lane >= 32;
+    float xs[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) xs[f] = __shfl_xor(x[f], 32);  // tile 1: samples 32..63 to lanes 0..31
+    MlpTile t0, t1;
+    mlp_tile_forward(w, lane, input_operand(x, upper), t0);
+    mlp_tile_forward(w, lane, input_operand(xs, upper), t1);
+    float a0 = __shfl_xor(t1.o[0], 32), a1 = __shfl_xor(t1.o[1], 32);
+    y[0] = upper ? a0 : t0.o[0];
+    y[1] = upper ? a1 : t0.o[1];
+}
+
+// ------------------------------------------------------------------ features and targets
+// Scaled inputs from the conserved state (D, S, tau, B): five O(1) numbers.
+//   x0 = log10 D / 4, x1 = log10(tau / D) / 4, x2 = log10(|S| / D + 1e-10) / 4,
+//   x3 = log10(B^2 / D + 1e-10) / 4, x4 = S.B / (|S| |B|)
+// Outputs: y0 = log10(p / D) / 4, y1 = log10 W.
+AFD_D bool mlp_features(const double *U, float x[5]) {
+    double D = U[0], tau = U[4];
+    double S2 = U[1] * U[1] + U[2] * U[2] + U[3] * U[3];
+    double B2 = U[5] * U[5] + U[6] * U[6] + U[7] * U[7];
+    double SB = U[1] * U[5] + U[2] * U[6] + U[3] * U[7];
+    bool ok = D > 0.0 && tau > 0.0 && isfinite(D) && isfinite(tau) && isfinite(S2) && isfinite(B2);
+    if (!ok) {
+        for (int f = 0; f < 5; ++f) x[f] = 0.0f;
+        return false;
+    }
+    const float l10 = 0.30102999566f;  // log10(2)
+    float Df = (float)D;
+    float Sm = (float)sqrt(S2);
+    x[0] = __log2f(Df) * l10 * 0.25f;
+    x[1] = __log2f((float)(tau / D)) * l10 * 0.25f;
+    x[2] = __log2f((float)(sqrt(S2) / D) + 1e-10f) * l10 * 0.25f;
+    x[3] = __log2f((float)(B2 / D) + 1e-10f) * l10 * 0.25f;
+    float den = Sm * (float)sqrt(B2);
+    x[4] = den > 0.0f ? (float)SB / den : 0.0f;
+    for (int f = 0; f < 5; ++f)
+        if (!isfinite(x[f])) ok = false;
+    return ok;
+}
+
+// Z0 = rho h W^2 = D W + Gamma/(Gamma-1) p W^2 from the network outputs
+AFD_D double mlp_guess_Z(double D, double gfac, const float y[2]) {
+    double p = D * exp10((double)(4.0f * y[0]));
+    double W = exp10((double)y[1]);
+    if (!(W >= 1.0)) W = 1.0;
+    return D * W + gfac * p * W * W;
+}
+
+}  // namespace afd

@@ -269,3 +269,49 @@ class Domain:
         L.check(self.lib.afd_mlp_get_weights(self.h, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), w.size),
                 "afd_mlp_get_weights")
         return w
+
+    def mlp_dataset_from_prims(self, prim):
+        """(features, targets, valid) of the con2prim pairs of primitive states (n x 9)."""
+        prim = L.f64c(prim).reshape(-1, 9)
+        n = prim.shape[0]
+        X = np.empty((n, 5), dtype=np.float32)
+        Y = np.empty((n, 2), dtype=np.float32)
+        valid = np.empty(n, dtype=np.int32)
+        L.check(self.lib.afd_mlp_dataset_from_prims(self.h, n, L.dptr(prim), _fptr(X), _fptr(Y),
+                                                    valid.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+                "afd_mlp_dataset_from_prims")
+        return X, Y, valid.astype(bool)
+
+    def mlp_set_dataset(self, X, Y):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 5)
+        Y = np.ascontiguousarray(Y, dtype=np.float32).reshape(-1, 2)
+        if X.shape[0] != Y.shape[0]:
+            raise ValueError("X and Y hold different numbers of samples")
+        L.check(self.lib.afd_mlp_set_dataset(self.h, X.shape[0], _fptr(X), _fptr(Y)), "afd_mlp_set_dataset")
+
+    def mlp_train(self, steps, batch, lr):
+        """`steps` Adam steps on consecutive minibatches; returns each minibatch's loss."""
+        hist = np.zeros(max(int(steps), 1))
+        L.check(self.lib.afd_mlp_train(self.h, int(steps), int(batch), float(lr), L.dptr(hist)), "afd_mlp_train")
+        return hist[:int(steps)]
+
+    def mlp_forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 5)
+        Y = np.empty((X.shape[0], 2), dtype=np.float32)
+        L.check(self.lib.afd_mlp_forward(self.h, X.shape[0], _fptr(X), _fptr(Y)), "afd_mlp_forward")
+        return Y
+
+    def mlp_loss_grad(self, X, Y):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 5)
+        Y = np.ascontiguousarray(Y, dtype=np.float32).reshape(-1, 2)
+        n = ctypes.c_int32()
+        L.check(self.lib.afd_mlp_param_count(ctypes.byref(n)), "afd_mlp_param_count")
+        g = np.empty(n.value, dtype=np.float32)
+        loss = np.zeros(1)
+        L.check(self.lib.afd_mlp_loss_grad(self.h, X.shape[0], _fptr(X), _fptr(Y), _fptr(g), L.dptr(loss)),
+                "afd_mlp_loss_grad")
+        return float(loss[0]), g
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))

@@ -161,8 +161,20 @@ int afd_synchronize(afd_ctx *ctx);
  * on scaled (D, |S|, tau, B^2, S.B) giving (log p, log W); BF16 weights, FP32 accumulation
  * on MFMA.  Weights are the packed FP32 master copy (4674 values, layout in afd_mlp_layout). */
 int afd_mlp_param_count(int32_t *count);
-int afd_mlp_set_weights(afd_ctx *ctx, const float *w, int32_t count);
+int afd_mlp_set_weights(afd_ctx *ctx, const float *w, int32_t count);  /* also resets the Adam state */
 int afd_mlp_get_weights(afd_ctx *ctx, float *w, int32_t count);
+/* Training pairs from primitive states (n x 9): the context's p2c, then its own FP64 con2prim;
+ * X = 5 scaled features of the conserved state, Y = (log10(p/D)/4, log10 W) of the con2prim
+ * output; valid = 1 only where the con2prim converged by Newton to a valid state (no bisection,
+ * no fallback to the hydro guess) and every feature and target is finite. */
+int afd_mlp_dataset_from_prims(afd_ctx *ctx, int64_t n, const double *prim, float *X, float *Y, int32_t *valid);
+int afd_mlp_set_dataset(afd_ctx *ctx, int64_t n, const float *X, const float *Y);
+/* steps of minibatch Adam (lr, betas 0.9/0.999) on consecutive minibatches of the dataset;
+ * loss_history (optional, steps): mean squared error of each minibatch before its update. */
+int afd_mlp_train(afd_ctx *ctx, int64_t steps, int32_t batch, float lr, double *loss_history);
+/* checks: network outputs, and loss + gradient w.r.t. the FP32 master layout, for host data */
+int afd_mlp_forward(afd_ctx *ctx, int64_t n, const float *X, float *Y);
+int afd_mlp_loss_grad(afd_ctx *ctx, int64_t n, const float *X, const float *Y, float *grad, double *loss);
 
 const char *afd_last_error(void);
 const char *afd_version(void);

@@ -71,6 +71,7 @@ def lib():
             "orc_srhd_flux_batch": [P, _I64, ctypes.c_int, _D, _D],
             "orc_ppm": [_D, _D], "orc_srhd_riemann": [P, ctypes.c_int, _D, _D, _D],
             "orc_rmhd_p2c_batch": [P, _I64, _D, _D], "orc_rmhd_c2p_batch": [P, _I64, _D, _D],
+            "orc_rmhd_c2p_flags_batch": [P, _I64, _D, _D, ctypes.POINTER(ctypes.c_int32)],
             "orc_rmhd_hydro_guess_batch": [P, _I64, _D, _D], "orc_rmhd_f_of_Z_batch": [P, _I64, _D, _D, _D],
             "orc_rmhd_fast_speed_batch": [P, _I64, _D, _D],
             "orc_rmhd_riemann": [P, ctypes.c_int, _D, _D, ctypes.c_int, _D],
@@ -191,6 +192,14 @@ def rmhd_p2c(P, prim):
 def rmhd_c2p(P, cons):
     cons = _c(cons); out = np.empty_like(cons)
     lib().orc_rmhd_c2p_batch(ctypes.byref(P), cons.shape[0], _p(cons), _p(out)); return out
+
+
+def rmhd_c2p_flags(P, cons):
+    """(prim, flags): flags 2 = bisection, 4 = invalid recovery (hydro guess returned)."""
+    cons = _c(cons); out = np.empty_like(cons); fl = np.zeros(cons.shape[0], dtype=np.int32)
+    lib().orc_rmhd_c2p_flags_batch(ctypes.byref(P), cons.shape[0], _p(cons), _p(out),
+                                   fl.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return out, fl
 
 
 def rmhd_hydro_guess(P, cons5):

@@ -682,8 +682,10 @@ static double rmhd_f_of_Z(const orc_params *P, double Z, double D, double tau, d
     return tau_calc - tau;
 }
 
-/* core/rmhd_core.py:248-341 */
-static void rmhd_c2p(const orc_params *P, const double *U, double *q) {
+/* core/rmhd_core.py:248-341.  Returns flags: 2 = Newton failed from Z0 and Z1 (bisection),
+ * 4 = invalid recovery (the hydro guess is returned). */
+static int rmhd_c2p(const orc_params *P, const double *U, double *q) {
+    int flags = 0;
     const double g = P->gamma;
     double D = U[0], Sx = U[1], Sy = U[2], Sz = U[3], tau = U[4], Bx = U[5], By = U[6], Bz = U[7], psi = U[8];
     double B2 = Bx * Bx + By * By + Bz * Bz;
@@ -723,6 +725,7 @@ static void rmhd_c2p(const orc_params *P, const double *U, double *q) {
         if (ok) break;
     }
     if (!ok) {
+        flags |= 2;
         double Zmin = SMALL;
         double Zmax = pymax(pymax(Z, Z1), 1.0);
         double fmin = rmhd_f_of_Z(P, Zmin, D, tau, B2, SB, S2);
@@ -767,11 +770,13 @@ static void rmhd_c2p(const orc_params *P, const double *U, double *q) {
     double h = Z / pymax(rho * W * W, SMALL);
     double p = (h - 1.0) * rho * (g - 1.0) / g;
     if (!isfinite(rho) || !isfinite(p) || rho <= 0.0 || p <= 0.0) {
+        flags |= 4;
         rho = q0[0]; vx = q0[1]; vy = q0[2]; vz = q0[3]; p = q0[4];
     }
     q[0] = rho; q[1] = vx; q[2] = vy; q[3] = vz; q[4] = p;
     q[5] = Bx; q[6] = By; q[7] = Bz; q[8] = psi;
     rmhd_floor(P->p_max, P->v_max, q);
+    return flags;
 }
 
 /* core/rmhd_core.py:419-457 */
@@ -950,6 +955,10 @@ void orc_rmhd_p2c_batch(const orc_params *P, int64_t n, const double *prim, doub
 void orc_rmhd_c2p_batch(const orc_params *P, int64_t n, const double *cons, double *prim) {
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t m = 0; m < n; ++m) rmhd_c2p(P, cons + 9 * m, prim + 9 * m);
+}
+
+void orc_rmhd_c2p_flags_batch(const orc_params *P, int64_t n, const double *cons, double *prim, int32_t *flags) {
+    for (int64_t m = 0; m < n; ++m) flags[m] = rmhd_c2p(P, cons + 9 * m, prim + 9 * m);
 }
 void orc_rmhd_hydro_guess_batch(const orc_params *P, int64_t n, const double *cons5, double *prim5) {
     for (int64_t m = 0; m < n; ++m) hydro_c2p(P->gamma, P->p_max, P->v_max, cons5 + 5 * m, prim5 + 5 * m);
@@ -1138,7 +1147,7 @@ void orc_step(const orc_params *P, const double *pr, double *out, double dx, dou
                 Uc[v * N + c] = uc;
                 U[v] = uc;
             }
-            if (nb == 9) rmhd_c2p(P, U, q);
+            if (nb == 9) (void)rmhd_c2p(P, U, q);
             else hydro_c2p(P->gamma, P->p_max, P->v_max, U, q);
             for (int v = 0; v < nb; ++v) dst[v * N + c] = q[v];
             for (int v = nb; v < P->nv; ++v) dst[v * N + c] = 0.0;

@@ -54,6 +54,8 @@ void orc_srhd_riemann(const orc_params *P, int dir, const double *primL, const d
 
 void orc_rmhd_p2c_batch(const orc_params *P, int64_t n, const double *prim, double *cons);
 void orc_rmhd_c2p_batch(const orc_params *P, int64_t n, const double *cons, double *prim);
+/* flags per state: 2 = bisection (Newton failed from both starts), 4 = invalid recovery */
+void orc_rmhd_c2p_flags_batch(const orc_params *P, int64_t n, const double *cons, double *prim, int32_t *flags);
 void orc_rmhd_hydro_guess_batch(const orc_params *P, int64_t n, const double *cons5, double *prim5);
 void orc_rmhd_f_of_Z_batch(const orc_params *P, int64_t n, const double *Z, const double *cons, double *f);
 void orc_rmhd_fast_speed_batch(const orc_params *P, int64_t n, const double *prim, double *cf);
