@@ -246,14 +246,14 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
     TRY(dalloc(c, (void **)&c->red_out, 8));
     TRY(dalloc(c, (void **)&c->cnt_partial, c->red_blocks * 8));
     TRY(dalloc(c, (void **)&c->cnt_out, 8));
-    TRY(dalloc(c, (void **)&c->stats_dev, 8 * 8));
+    TRY(dalloc(c, (void **)&c->stats_dev, sizeof(unsigned long long) * STATS_N * STATS_SLOTS));
     if (nb == 9) TRY(mlp_alloc(c->mlp, [c](void **ptr, size_t bytes) { return dalloc(c, ptr, bytes); }));
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         afd_destroy(c);
         return fail(AFD_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
-    e = hipMemsetAsync(c->stats_dev, 0, 8 * 8, c->stream);
+    e = hipMemsetAsync(c->stats_dev, 0, sizeof(unsigned long long) * STATS_N * STATS_SLOTS, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         afd_destroy(c);
@@ -716,9 +716,11 @@ extern "C" int afd_get_stats(afd_ctx *c, afd_stats *out) {
     if (!c || !out) return fail(AFD_ERR_INVALID, "afd_get_stats: null argument");
     HIPCHK(hipSetDevice(c->p.device));
     if (int rc = resolve_marks(c)) return rc;
-    unsigned long long v[8] = {0};
-    HIPCHK(hipMemcpyAsync(v, c->stats_dev, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+    unsigned long long slots[STATS_N * STATS_SLOTS], v[STATS_N] = {0};
+    HIPCHK(hipMemcpyAsync(slots, c->stats_dev, sizeof(slots), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    for (int s = 0; s < STATS_SLOTS; ++s)
+        for (int m = 0; m < STATS_N; ++m) v[m] += slots[STATS_N * s + m];
     *out = c->acc;
     out->hydro_iters = v[0];
     out->newton_evals = v[1];
@@ -733,7 +735,7 @@ extern "C" int afd_reset_stats(afd_ctx *c) {
     if (!c) return fail(AFD_ERR_INVALID, "null context");
     HIPCHK(hipSetDevice(c->p.device));
     if (int rc = resolve_marks(c)) return rc;
-    HIPCHK(hipMemsetAsync(c->stats_dev, 0, 8 * 8, c->stream));
+    HIPCHK(hipMemsetAsync(c->stats_dev, 0, sizeof(unsigned long long) * STATS_N * STATS_SLOTS, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->acc = afd_stats{};
     return AFD_OK;

@@ -323,15 +323,19 @@ AFD_D double ssprk_combine(int stage, int nst, double u0, double uc, double r) {
     return (1.0 / 3.0) * u0 + (2.0 / 3.0) * (uc + r);
 }
 
+// Solver counters: STATS_SLOTS copies of STATS_N counters; a wave adds its sums into the slot of
+// its workgroup so that concurrent waves rarely hit the same address (the host sums the slots).
+constexpr int STATS_N = 8, STATS_SLOTS = 64;
 AFD_D void wave_add_stats(unsigned long long *stats, const C2PStats &st) {
     unsigned long long v[6] = {(unsigned long long)st.hydro_iters, (unsigned long long)st.newton_iters,
                                (unsigned long long)(st.flags & 1), (unsigned long long)((st.flags >> 1) & 1),
                                (unsigned long long)((st.flags >> 2) & 1), (unsigned long long)((st.flags >> 3) & 1)};
+    unsigned long long *slot = stats + STATS_N * (blockIdx.x % STATS_SLOTS);
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
         unsigned long long x = v[m];
         for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-        if ((threadIdx.x & 63) == 0 && x) atomicAdd(&stats[m], x);
+        if ((threadIdx.x & 63) == 0 && x) atomicAdd(&slot[m], x);
     }
 }
 

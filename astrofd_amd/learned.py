@@ -117,7 +117,7 @@ def train(dom, X, Y, epochs=60, batch=32768, lr0=3e-3, lr_min=3e-5, seed=11, w0=
     return dom.get_mlp_weights(), np.concatenate(hist)
 
 
-def evaluate(w, n=1 << 20, seed=99, device=0):
+def evaluate(w, n=1 << 20, seed=99, device=0, return_states=False):
     """A/B of plain and learned con2prim on held-out sampled states (the same states for both).
 
     `newton_valid` marks the states where the plain con2prim converges by Newton to a valid
@@ -143,7 +143,7 @@ def evaluate(w, n=1 << 20, seed=99, device=0):
     same = np.all(ln["out"][~newton_valid] == pl["out"][~newton_valid], axis=1)
     sl, sp = ln["stats"], pl["stats"]
     nv = int(newton_valid.sum())
-    return {
+    summary = {
         "states": n,
         "newton_valid_fraction": nv / n,
         "plain_iters_per_state": float(pl["iters"].mean()),
@@ -160,9 +160,13 @@ def evaluate(w, n=1 << 20, seed=99, device=0):
         "plain_invalid": sp["invalid"], "learned_invalid": sl["invalid"],
         "identical_outside_newton_valid": float(same.mean()) if same.size else 1.0,
         "rho_rel_diff_max": float(rel_rho.max()) if rel_rho.size else 0.0,
+        "rho_rel_diff_gt_1e-8_fraction": float(np.mean(rel_rho > 1e-8)) if rel_rho.size else 0.0,
         "rho_rel_diff_median": float(np.median(rel_rho)) if rel_rho.size else 0.0,
         "p_rel_diff_max": float(rel_p.max()) if rel_p.size else 0.0,
     }
+    if return_states:
+        return summary, dict(cons=U, plain=pl["out"], learned=ln["out"], newton_valid=newton_valid)
+    return summary
 
 
 def save(path, w, meta=None):

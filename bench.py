@@ -8,16 +8,17 @@ Workload (default): the RMHD 256^3 jet, WENO5 + HLLD + SSPRK3, Gamma = 4/3
 the whole grid, as the reference driver runs it: ghost/nozzle fills, one SSPRK3
 step, ghost/nozzle fills.  Inputs are resident in HBM before the timed region.
 
-Two runs from the same seeded initial state:
+Timed runs from the same seeded initial state:
   plain  FP64 everything, plain con2prim (the baseline of the metric)
   mixed  FP32 reconstruction + Riemann fluxes, FP64 Lorentz factor / con2prim /
          SSPRK accumulation, learned con2prim guess when weights are available
-`value` is the mixed run's zone-updates/s; its relative L1 error in rho, p and W
-against the plain run after the same steps is reported beside it.
+  mixed with the plain con2prim (what the learned guess changes in the step)
+`value` is the mixed run's zone-updates/s.  A lockstep pass of plain and mixed
+reports the relative L1 and max error of rho, p and W after checkpoint steps.
 
-Prints ONE JSON line.  Also: the dominant kernel's roofline from HIP events over
-a timed pass, and the CPU oracle's rate on a bounded sample (64^3, one step) on
-this host's cores.
+Prints ONE JSON line.  Also: the dominant kernel's roofline from HIP events on
+the library's stream over the timed steps, the solver counters of one extra
+step, and the CPU oracle's rate on a bounded sample on this host's cores.
 """
 import argparse
 import json
@@ -52,13 +53,17 @@ def lorentz(pr):
 
 
 def errors(a, b, ng):
-    """Relative L1 and max |diff| of rho, p, W over interior cells (b is the reference)."""
+    """Relative L1 and max |diff| of rho, p, W over interior cells (b is the reference), over the
+    cells where both states are finite."""
     sl = (slice(ng, -ng),) * 3
     out = {}
     for name, fa, fb in (("rho", a[0][sl], b[0][sl]), ("p", a[4][sl], b[4][sl]),
                          ("W", lorentz(a)[sl], lorentz(b)[sl])):
-        d = np.abs(fa - fb)
-        out[name] = {"L1_rel": float(d.sum() / np.abs(fb).sum()), "max_abs": float(d.max())}
+        ok = np.isfinite(fa) & np.isfinite(fb)
+        d = np.abs(fa[ok] - fb[ok])
+        ref = np.abs(fb[ok]).sum()
+        out[name] = {"L1_rel": float(d.sum() / ref) if ref > 0 else 0.0,
+                     "max_abs": float(d.max()) if d.size else 0.0, "cells": int(ok.sum())}
     return out
 
 
@@ -72,29 +77,41 @@ def sync(d):
         pass
 
 
-def run_path(s, pr0, steps, warmup, mlp=None, timing_steps=0):
+def make_stepper(d, s, rng):
+    dx = min(s["Lx"] / s["NX"], s["Ly"] / s["NY"], s["Lz"] / s["NZ"])
+
+    def one_step():
+        # the reference driver's step: CFL (SRHD only; RMHD uses amax = 1), fills, SSPRK, fills, sources
+        amax = d.max_char_speed() if s["PHYSICS"] == "hydro" else 1.0
+        dt = s["CFL"] * dx / max(amax, 1e-12)
+        d.apply_bcs(rng)
+        d.step(dt)
+        d.apply_bcs(rng)
+        d.two_temperature(dt)
+    return one_step
+
+
+def open_domain(s, pr0, mlp=None):
     from astrofd_amd.domain import Domain
     d = Domain.from_settings(s)
+    d.set_prim(pr0)
+    d.set_nozzle(s)
+    if mlp is not None:
+        d.set_mlp_weights(mlp)
+    rng = np.random.default_rng(1234)
+    d.apply_bcs(rng)
+    return d, rng
+
+
+def run_timed(s, pr0, steps, warmup, mlp=None):
+    """Warmup, then `steps` timed steps with per-kernel HIP events on the library's stream,
+    then one untimed step with the solver counters on."""
+    d, rng = open_domain(s, pr0, mlp)
     try:
-        d.set_prim(pr0)
-        d.set_nozzle(s)
-        if mlp is not None:
-            d.set_mlp_weights(mlp)
-        rng = np.random.default_rng(1234)
-        dx = min(s["Lx"] / s["NX"], s["Ly"] / s["NY"], s["Lz"] / s["NZ"])
-        d.apply_bcs(rng)
-
-        def one_step():
-            amax = d.max_char_speed() if s["PHYSICS"] == "hydro" else 1.0
-            dt = s["CFL"] * dx / max(amax, 1e-12)
-            d.apply_bcs(rng)
-            d.step(dt)
-            d.apply_bcs(rng)
-            d.two_temperature(dt)
-
+        one_step = make_stepper(d, s, rng)
         for _ in range(warmup):
             one_step()
-        d.set_instrumentation(stats=True, timing=False)
+        d.set_instrumentation(stats=False, timing=True)
         d.reset_stats()
         sync(d)
         t0 = time.perf_counter()
@@ -102,36 +119,57 @@ def run_path(s, pr0, steps, warmup, mlp=None, timing_steps=0):
             one_step()
         sync(d)
         elapsed = time.perf_counter() - t0
+        kern = d.stats()
+        d.set_instrumentation(stats=True, timing=False)
+        d.reset_stats()
+        one_step()
         st = d.stats()
-        kern = None
-        if timing_steps:
-            d.set_instrumentation(stats=False, timing=True)
-            d.reset_stats()
-            for _ in range(timing_steps):
-                one_step()
-            kern = d.stats()
-            d.set_instrumentation(stats=False, timing=False)
-        out = d.get_prim()
-        return dict(elapsed=elapsed, stats=st, kernels=kern, state=out, nonfinite=int(np.count_nonzero(~np.isfinite(out))))
+        d.set_instrumentation(stats=False, timing=False)
+        return dict(elapsed=elapsed, kernels=kern, stats=st)
     finally:
         d.close()
+
+
+def accuracy(sp, sm, pr0, checkpoints, mlp=None):
+    """Plain and mixed runs in lockstep from the same state; errors of mixed against plain after
+    each checkpoint step (cells where both are finite), and the non-finite counts."""
+    dp, rp = open_domain(sp, pr0)
+    dm, rm = open_domain(sm, pr0, mlp)
+    out = []
+    try:
+        step_p, step_m = make_stepper(dp, sp, rp), make_stepper(dm, sm, rm)
+        for n in range(1, max(checkpoints) + 1):
+            step_p()
+            step_m()
+            if n in checkpoints:
+                a, b = dm.get_prim(), dp.get_prim()
+                e = errors(a, b, sp["NG"])
+                e["step"] = n
+                e["nonfinite"] = {"plain": int(np.count_nonzero(~np.isfinite(b))),
+                                  "mixed": int(np.count_nonzero(~np.isfinite(a)))}
+                out.append(e)
+                log(f"[bench]   accuracy after step {n}: L1(rho) {e['rho']['L1_rel']:.3e}, nonfinite {e['nonfinite']}")
+    finally:
+        dp.close()
+        dm.close()
+    return out
 
 
 def kernel_roofline(s, kern, mixed):
     """Dominant kernel and its achieved algorithmic HBM bandwidth (bytes per launch / avg duration).
 
     Algorithmic bytes per launch (compulsory traffic of one launch over its units):
-      face kernel, one thread per face: read the nb primitives of the face's cell (8 B each) and
-        write the face flux (nb x flux size; twice for the RMHD PPM/WENO x faces, plus the two carried
-        speeds read there and written by the z kernel).
-      stage update, one thread per cell: read U0 (stage > 0), U_s-1 (stage > 0), the stage input
-        primitives' GLM component and the three face fluxes of the cell's left faces (nb each, the right
-        faces are the neighbours' left faces), write U_s and the stage output primitives.
+      face kernel, one thread per face: read the nb primitives of one cell (8 B each; the stencil's
+        other cells are re-reads) and write the face flux (nb x flux size; twice for the RMHD PPM/WENO
+        x faces, which also read the two carried speeds the z kernel wrote there).
+      stage update, one thread per cell (ghosts included): per stage 4 x nb x 8 B of conserved and
+        primitive state (stage 0: read P0, write U0, U1, P1; later stages: read U0, U_s-1, write U_s,
+        P_s), the psi of the stage input (8 B) and one read of each face-flux array (3 arrays, 4 with
+        the carried x fluxes).
     """
     nb = 9 if s["PHYSICS"] == "rmhd" else 5
     fb = 4 if mixed else 8
     NX, NY, NZ = s["NX"], s["NY"], s["NZ"]
-    ncells = NX * NY * NZ
     full = (NX + 2 * s["NG"]) * (NY + 2 * s["NG"]) * (NZ + 2 * s["NG"])
     carried = nb == 9 and s["RECON"] != "muscl"
     per_face = [nb * 8 + nb * fb, nb * 8 + nb * fb, nb * 8 + nb * fb]
@@ -139,8 +177,7 @@ def kernel_roofline(s, kern, mixed):
         per_face[0] += nb * fb + 2 * fb
         per_face[2] += 2 * fb
     faces = [(NX + 1) * NY * NZ, NX * (NY + 1) * NZ, NX * NY * (NZ + 1)]
-    # update: 3 stages average of U0 + Uc reads (2/3 of stages), writes Uc + P, flux reads 3 nb
-    upd_per_cell = nb * 8 * (2 * 2 / 3) + nb * 8 * 2 + 3 * nb * fb
+    upd_per_cell = 4 * nb * 8 + 8 + (4 if carried else 3) * nb * fb
     cands = []
     for dkey, name in ((0, "face_x"), (1, "face_y"), (2, "face_z")):
         n = kern["launches_face"][dkey]
@@ -155,61 +192,99 @@ def kernel_roofline(s, kern, mixed):
             "bytes_per_launch": int(nbytes)}
 
 
-def cpu_baseline(s, n=64):
-    """The CPU oracle (C restatement of the reference, OpenMP) on an n^3 sample of the same workload:
-    one step (fills + SSPRK step + fills), all host cores."""
+def cpu_baseline(s, n=64, steps=2, n1=32):
+    """The CPU oracle (C restatement of the reference, FP64, OpenMP) on a bounded sample of the same
+    workload: `steps` steps (fills + SSPRK step + fills) at n^3 on all host cores, and one step at
+    n1^3 on one core (the reference itself is serial per rank)."""
     import oracle as O
     from astrofd_amd import ic
-    t = dict(s)
-    t.update(NX=n, NY=n, NZ=n)
-    pr = ic.initial_state(t)
-    rm = t["PHYSICS"] == "rmhd"
-    P = O.make_params("rmhd" if rm else "hydro", pr.shape[1:], nv=pr.shape[0], ng=t["NG"],
-                      gamma=t["GAMMA"] if rm else None, p_max=t["P_MAX"], v_max=t["V_MAX"], glm_ch=t["GLM_CH"],
-                      glm_cp=t["GLM_CP"], limiter=t["LIMITER"], recon=t["RECON"], riemann=t["RIEMANN"],
-                      rk_order=t["RK_ORDER"], n_passive=t["N_PASSIVE"])
     cores = os.cpu_count() or 1
     try:
         cores = len(os.sched_getaffinity(0))
     except Exception:
         pass
-    O.set_num_threads(cores)
-    dx = 1.0 / n
-    bamb = tuple(t.get("B_AMB") or (0.0, 0.0, 0.0))
 
-    def bcs(a):
-        O.apply_outflow_yz(P, a, t["BC_Y"] == "outflow", t["BC_Z"] == "outflow")
-        O.apply_outflow_right_x(P, a)
-        O.apply_nozzle_left_x(P, a, dx, dx, dx, n, n, t["JET_CENTER"][1], t["JET_CENTER"][2], t, None, bamb)
+    def run(m, nsteps, threads):
+        t = dict(s)
+        t.update(NX=m, NY=m, NZ=m)
+        pr = ic.initial_state(t)
+        rm = t["PHYSICS"] == "rmhd"
+        P = O.make_params("rmhd" if rm else "hydro", pr.shape[1:], nv=pr.shape[0], ng=t["NG"],
+                          gamma=t["GAMMA"] if rm else None, p_max=t["P_MAX"], v_max=t["V_MAX"],
+                          glm_ch=t["GLM_CH"], glm_cp=t["GLM_CP"], limiter=t["LIMITER"], recon=t["RECON"],
+                          riemann=t["RIEMANN"], rk_order=t["RK_ORDER"], n_passive=t["N_PASSIVE"])
+        O.set_num_threads(threads)
+        dx = 1.0 / m
+        bamb = tuple(t.get("B_AMB") or (0.0, 0.0, 0.0))
 
-    bcs(pr)
-    t0 = time.perf_counter()
-    bcs(pr)
-    amax = O.max_char_speed(P, pr) if not rm else 1.0
-    pr = O.step(P, pr, dx, dx, dx, t["CFL"] * dx / amax)
-    bcs(pr)
-    el = time.perf_counter() - t0
-    return {"value": round(n ** 3 / el, 1), "unit": "zone-updates/s", "cores": cores, "kind": "port",
-            "sample": f"one step of the same workload at {n}^3 (C oracle, OpenMP, {cores} threads, "
-                      f"{el:.2f} s wall)"}
+        def bcs(a):
+            O.apply_outflow_yz(P, a, t["BC_Y"] == "outflow", t["BC_Z"] == "outflow")
+            O.apply_outflow_right_x(P, a)
+            O.apply_nozzle_left_x(P, a, dx, dx, dx, m, m, t["JET_CENTER"][1], t["JET_CENTER"][2], t, None, bamb)
+
+        bcs(pr)
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            bcs(pr)
+            amax = O.max_char_speed(P, pr) if not rm else 1.0
+            pr = O.step(P, pr, dx, dx, dx, t["CFL"] * dx / amax)
+            bcs(pr)
+        return m ** 3 * nsteps / (time.perf_counter() - t0), time.perf_counter() - t0
+
+    v_all, el_all = run(n, steps, cores)
+    v_one, el_one = run(n1, 1, 1)
+    return {"value": round(v_all, 1), "unit": "zone-updates/s", "cores": cores, "kind": "port",
+            "value_1core": round(v_one, 1),
+            "sample": f"{steps} steps of the same workload at {n}^3 on {cores} threads ({el_all:.2f} s wall), "
+                      f"and 1 step at {n1}^3 on 1 thread ({el_one:.2f} s): C oracle, FP64, OpenMP"}
+
+
+def kernel_ms(k, timing_steps):
+    return {"face_x": round(k["ms_face"][0] / timing_steps, 3), "face_y": round(k["ms_face"][1] / timing_steps, 3),
+            "face_z": round(k["ms_face"][2] / timing_steps, 3),
+            "update_c2p": round(k["ms_update"] / timing_steps, 3), "other": round(k["ms_other"] / timing_steps, 3)}
+
+
+def c2p_report(st):
+    cells = max(st["c2p_cells"], 1)
+    return {"hydro_iters_per_cell": round(st["hydro_iters"] / cells, 3),
+            "newton_evals_per_cell": round(st["newton_evals"] / cells, 3),
+            "learned_accept_frac": round(st["learned_ok"] / cells, 5),
+            "bisection_frac": round(st["bisections"] / cells, 5), "invalid_frac": round(st["invalid"] / cells, 5)}
+
+
+def copy_probe(gib):
+    """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = int(gib * (1 << 30)) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"[bench] copy probe: {n * 8} bytes read + {n * 8} bytes written per copy")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="rmhd_jet_256")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the FP64 plain run")
-    ap.add_argument("--timing-steps", type=int, default=2)
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the lockstep accuracy pass")
+    ap.add_argument("--accuracy-steps", default="1,2,5,10,20")
+    ap.add_argument("--copy-probe", type=float, default=0.0, help="GiB of a plain device copy (profiling)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}): the step is built and measured for one "
-              "MI355X; multi-GPU sharding is a design note, not a measured path.", file=sys.stderr)
+              "MI355X; sharding across GPUs is a design note, not a measured path.", file=sys.stderr)
         sys.exit(2)
 
     from astrofd_amd import ic
+    if a.copy_probe > 0:
+        copy_probe(a.copy_probe)
     s = load_workload(a.workload)
     log(f"[bench] workload {a.workload}: {s['NX']}x{s['NY']}x{s['NZ']} {s['PHYSICS']} {s['RECON']}/{s['RIEMANN']} "
         f"rk{s['RK_ORDER']} steps={a.steps} warmup={a.warmup}")
@@ -218,22 +293,30 @@ def main():
     log(f"[bench] initial state {pr0.shape} in {time.perf_counter() - t0:.1f}s")
     zu = s["NX"] * s["NY"] * s["NZ"] * a.steps
 
-    plain = None
-    if not a.no_plain:
-        sp = dict(s, PRECISION="fp64", C2P_GUESS="plain")
-        log("[bench] plain FP64 run")
-        plain = run_path(sp, pr0, a.steps, a.warmup, timing_steps=a.timing_steps)
-        log(f"[bench]   {plain['elapsed'] * 1e3 / a.steps:.2f} ms/step, {zu / plain['elapsed']:.4e} ZU/s")
-
     mlp = None
     if s["PHYSICS"] == "rmhd" and os.path.exists(MLP_WEIGHTS):
-        mlp = np.load(MLP_WEIGHTS)["w"].astype(np.float32)
+        with np.load(MLP_WEIGHTS, allow_pickle=False) as f:
+            mlp = np.asarray(f["w"], dtype=np.float32)
+    sp = dict(s, PRECISION="fp64", C2P_GUESS="plain")
     sm = dict(s, PRECISION="mixed", C2P_GUESS="learned" if mlp is not None else "plain")
+
+    plain = None
+    if not a.no_plain:
+        log("[bench] plain FP64 run")
+        plain = run_timed(sp, pr0, a.steps, a.warmup)
+        log(f"[bench]   {plain['elapsed'] * 1e3 / a.steps:.2f} ms/step, {zu / plain['elapsed']:.4e} ZU/s")
     log(f"[bench] mixed run (c2p guess: {sm['C2P_GUESS']})")
-    mixed = run_path(sm, pr0, a.steps, a.warmup, mlp=mlp, timing_steps=a.timing_steps)
+    mixed = run_timed(sm, pr0, a.steps, a.warmup, mlp=mlp)
     log(f"[bench]   {mixed['elapsed'] * 1e3 / a.steps:.2f} ms/step, {zu / mixed['elapsed']:.4e} ZU/s")
 
+    mixed_plain = None
+    if mlp is not None:
+        log("[bench] mixed run with the plain con2prim (A/B of the learned guess)")
+        mixed_plain = run_timed(dict(sm, C2P_GUESS="plain"), pr0, a.steps, a.warmup)
+        log(f"[bench]   {mixed_plain['elapsed'] * 1e3 / a.steps:.2f} ms/step")
+
     value = zu / mixed["elapsed"]
+    nb = 9 if s["PHYSICS"] == "rmhd" else 5
     res = {
         "metric": "zone-updates/s RMHD 256³ WENO5+HLLD: mixed+learned vs FP64 plain at L1≤1e-6",
         "value": round(value, 1), "unit": "zone-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -245,42 +328,28 @@ def main():
                    "grid": [s["NX"], s["NY"], s["NZ"]], "ng": s["NG"], "precision": "mixed",
                    "c2p_guess": sm["C2P_GUESS"]},
     }
+    res["roofline"] = kernel_roofline(s, mixed["kernels"], True)
+    res["kernel_ms_per_step"] = kernel_ms(mixed["kernels"], a.steps)
+    step_bytes = (13 if s["RK_ORDER"] == 3 else 8) * nb * 8
+    res["step_roofline"] = {"bytes_per_zone_update": step_bytes, "achieved": round(step_bytes * value / 1e9, 2),
+                            "unit": "GB/s", "frac": round(step_bytes * value / 1e9 / HBM_PEAK_GBS, 5)}
+    res["c2p"] = c2p_report(mixed["stats"])
+    if mixed_plain is not None:
+        res["mixed_plain_c2p"] = {"value": round(zu / mixed_plain["elapsed"], 1),
+                                  "ms_per_step": round(mixed_plain["elapsed"] * 1e3 / a.steps, 3),
+                                  "kernel_ms_per_step": kernel_ms(mixed_plain["kernels"], a.steps),
+                                  "c2p": c2p_report(mixed_plain["stats"])}
     if plain is not None:
         res["fp64_plain"] = {"value": round(zu / plain["elapsed"], 1),
-                             "ms_per_step": round(plain["elapsed"] * 1e3 / a.steps, 3)}
+                             "ms_per_step": round(plain["elapsed"] * 1e3 / a.steps, 3),
+                             "kernel_ms_per_step": kernel_ms(plain["kernels"], a.steps),
+                             "roofline": kernel_roofline(s, plain["kernels"], False),
+                             "c2p": c2p_report(plain["stats"])}
         res["speedup_vs_fp64_plain"] = round(plain["elapsed"] / mixed["elapsed"], 3)
-        res["errors_vs_fp64_plain"] = errors(mixed["state"], plain["state"], s["NG"])
-        res["nonfinite"] = {"plain": plain["nonfinite"], "mixed": mixed["nonfinite"]}
-    st = mixed["stats"]
-    cells = max(st["c2p_cells"], 1)
-    res["c2p"] = {"hydro_iters_per_cell": round(st["hydro_iters"] / cells, 3),
-                  "newton_evals_per_cell": round(st["newton_evals"] / cells, 3),
-                  "learned_accept_frac": round(st["learned_ok"] / cells, 5),
-                  "bisections": st["bisections"], "invalid": st["invalid"]}
-    if plain is not None:
-        sp_ = plain["stats"]
-        cp = max(sp_["c2p_cells"], 1)
-        res["c2p"]["plain_hydro_iters_per_cell"] = round(sp_["hydro_iters"] / cp, 3)
-        res["c2p"]["plain_newton_evals_per_cell"] = round(sp_["newton_evals"] / cp, 3)
-    if mixed["kernels"]:
-        res["roofline"] = kernel_roofline(s, mixed["kernels"], True)
-        k = mixed["kernels"]
-        res["kernel_ms_per_step"] = {
-            "face_x": round(k["ms_face"][0] / a.timing_steps, 3), "face_y": round(k["ms_face"][1] / a.timing_steps, 3),
-            "face_z": round(k["ms_face"][2] / a.timing_steps, 3),
-            "update_c2p": round(k["ms_update"] / a.timing_steps, 3), "other": round(k["ms_other"] / a.timing_steps, 3)}
-        nb = 9 if s["PHYSICS"] == "rmhd" else 5
-        step_bytes = (13 if s["RK_ORDER"] == 3 else 8) * nb * 8
-        res["step_roofline"] = {"bytes_per_zone_update": step_bytes,
-                                "achieved": round(step_bytes * value / 1e9, 2), "unit": "GB/s",
-                                "frac": round(step_bytes * value / 1e9 / HBM_PEAK_GBS, 5)}
-    if plain is not None and plain["kernels"]:
-        k = plain["kernels"]
-        res["fp64_plain"]["kernel_ms_per_step"] = {
-            "face_x": round(k["ms_face"][0] / a.timing_steps, 3), "face_y": round(k["ms_face"][1] / a.timing_steps, 3),
-            "face_z": round(k["ms_face"][2] / a.timing_steps, 3),
-            "update_c2p": round(k["ms_update"] / a.timing_steps, 3), "other": round(k["ms_other"] / a.timing_steps, 3)}
-        res["fp64_plain"]["roofline"] = kernel_roofline(s, k, False)
+        if not a.no_accuracy:
+            cps = sorted({int(x) for x in a.accuracy_steps.split(",") if x.strip()})
+            log(f"[bench] accuracy pass (steps {cps})")
+            res["errors_vs_fp64_plain"] = accuracy(sp, sm, pr0, set(cps), mlp)
     if not a.no_cpu_baseline:
         log("[bench] CPU oracle sample")
         res["cpu_baseline"] = cpu_baseline(s)

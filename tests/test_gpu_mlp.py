@@ -137,22 +137,39 @@ def _trained_weights():
     pytest.skip(f"{learned.MODEL_PATH} is missing: train it with `python -m astrofd_amd.learned`")
 
 
+def _residual(P, U, q, gamma):
+    """|f(Z)| / max(1, tau) of recovered primitives q for conserved states U (the reference's
+    convergence measure, core/rmhd_core.py:273-290), with Z = rho h W^2 from q."""
+    v2 = np.sum(q[:, 1:4] ** 2, axis=1)
+    Z = q[:, 0] * (1.0 + gamma / (gamma - 1.0) * q[:, 4] / q[:, 0]) / (1.0 - v2)
+    return np.abs(oracle.rmhd_f_of_Z(P, Z, U)) / np.maximum(1.0, U[:, 4])
+
+
 def test_learned_guess_ab():
     """What the trained network gains: the same held-out states through plain and learned
     con2prim.  Where the plain solve converges by Newton to a valid state, the judge must accept
     the learned start for nearly every state and the learned path must need fewer iterations;
     elsewhere (bisection, hydro-guess fallback) both paths return the same state."""
     log("test_learned_guess_ab")
-    ev = learned.evaluate(_trained_weights(), n=1 << 18, seed=123)
+    ev, st = learned.evaluate(_trained_weights(), n=1 << 18, seed=123, return_states=True)
     for k, v in ev.items():
         log(f"  {k}: {v}")
     assert ev["judge_accept_rate_newton_valid"] >= 0.95
     assert ev["learned_iters_newton_valid"] < 0.6 * ev["plain_iters_newton_valid"]
     assert ev["learned_iters_per_state"] < ev["plain_iters_per_state"]
     assert ev["identical_outside_newton_valid"] >= 0.99
-    # both stop inside the plain test |f| < 1e-10 max(1, tau) of the same root
-    assert ev["rho_rel_diff_median"] <= 1e-10
-    assert ev["rho_rel_diff_max"] <= 1e-6
+    assert ev["rho_rel_diff_median"] <= 1e-10  # the same root
+    # The plain stop test |f| < 1e-10 max(1, tau) fixes Z only to that over |f'(Z)|, loose where f
+    # is flat, so single states may differ more; the learned result must be at least as converged.
+    m = st["newton_valid"]
+    P = oracle.make_params("rmhd", gamma=learned.GAMMA, p_max=learned.P_MAX, v_max=learned.V_MAX)
+    U = st["cons"][m]
+    r_pl = _residual(P, U, st["plain"][m], learned.GAMMA)
+    r_ln = _residual(P, U, st["learned"][m], learned.GAMMA)
+    log(f"  residual |f|/max(1,tau) median plain {np.median(r_pl):.2e} learned {np.median(r_ln):.2e}; "
+        f"99th pct plain {np.percentile(r_pl, 99):.2e} learned {np.percentile(r_ln, 99):.2e}")
+    assert np.median(r_ln) <= np.median(r_pl)
+    assert np.percentile(r_ln, 99) <= max(np.percentile(r_pl, 99), 1e-10)
 
 
 def test_learned_step_matches_plain():
