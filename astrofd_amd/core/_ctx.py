@@ -1,0 +1,75 @@
+"""Device contexts behind the reference-named modules, cached per configuration and shape."""
+import numpy as np
+
+from .. import _lib as L
+from ..domain import Domain, params_from_settings
+from ..settings import finalize
+
+_SETTINGS = {"hydro": None, "rmhd": None}
+_CACHE = {}
+
+
+def configure(physics, params):
+    """Keep the settings dict for `physics` ("hydro" or "rmhd"), as the reference's configure()
+    freezes its module globals; drops contexts made under an earlier configuration."""
+    s = finalize(dict(params)) if "N_PASSIVE" not in params else dict(params)
+    _SETTINGS[physics] = s
+    for key in [k for k in _CACHE if k[0] == physics]:
+        _CACHE.pop(key).close()
+
+
+def settings(physics):
+    s = _SETTINGS[physics]
+    if s is None:
+        raise RuntimeError(f"configure(params) must run before the {physics} core is used "
+                           "(core/srhd_core.py:18, core/rmhd_core.py:21)")
+    return s
+
+
+def domain(physics, full_shape, dxyz=(1.0, 1.0, 1.0), rk_order=2, ng=None, gamma=None):
+    """A context for arrays of shape (nv, NX+2NG, NY+2NG, NZ+2NG)."""
+    s = settings(physics)
+    ng = int(s["NG"] if ng is None else ng)
+    nv, fx, fy, fz = (int(v) for v in full_shape)
+    key = (physics, nv, fx, fy, fz, ng, tuple(float(d) for d in dxyz), int(rk_order), gamma)
+    d = _CACHE.get(key)
+    if d is None:
+        s2 = dict(s, PHYSICS=physics)
+        over = dict(nx=fx - 2 * ng, ny=fy - 2 * ng, nz=fz - 2 * ng, ng=ng, dx=float(dxyz[0]), dy=float(dxyz[1]),
+                    dz=float(dxyz[2]), rk_order=int(rk_order), nv=nv)
+        nb = 9 if physics == "rmhd" else 5
+        over["n_passive"] = nv - nb
+        if gamma is not None:
+            over["gamma"] = float(gamma)
+        # NX..NZ only size the settings-derived dx; the overrides set the grid
+        s2.update(NX=max(fx - 2 * ng, 1), NY=max(fy - 2 * ng, 1), NZ=max(fz - 2 * ng, 1))
+        d = Domain(params_from_settings(s2, **over))
+        _CACHE[key] = d
+    return d
+
+
+def pointwise(physics, gamma=None):
+    """A small context for the pointwise conversions (its grid is unused)."""
+    return domain(physics, (9 if physics == "rmhd" else 5, 8, 8, 8), ng=2, gamma=gamma)
+
+
+def as_batch(args, nb):
+    """Scalars or arrays of one shape -> (n x nb float64 array, shape)."""
+    arrs = np.broadcast_arrays(*[np.asarray(a, dtype=np.float64) for a in args])
+    shape = arrs[0].shape
+    return np.ascontiguousarray(np.stack([a.ravel() for a in arrs], axis=1)), shape
+
+
+def unbatch(out, shape):
+    if shape == ():
+        return tuple(float(v) for v in out[0])
+    return tuple(out[:, m].reshape(shape) for m in range(out.shape[1]))
+
+
+def close_all():
+    for d in _CACHE.values():
+        d.close()
+    _CACHE.clear()
+
+
+__all__ = ["configure", "settings", "domain", "pointwise", "as_batch", "unbatch", "close_all", "L"]

@@ -78,7 +78,7 @@ def vector_potential_field(shape, kpeak, seed, amp):
     A = []
     for _ in range(3):
         modes = (rng.normal(size=K.shape) + 1j * rng.normal(size=K.shape)) * w
-        A.append(np.fft.irfftn(modes, s=shape))
+        A.append(np.fft.irfftn(modes, s=shape, axes=(0, 1, 2)))
     Ax, Ay, Az = A
 
     def d(f, axis, h):

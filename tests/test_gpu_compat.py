@@ -1,0 +1,128 @@
+"""GPU tests of the reference-named modules (astrofd_amd.core.*) and the driver against the CPU
+oracle and the 32^3 SRHD driver fixture: the same calls a reference user makes, with numpy in
+and numpy out."""
+import ast
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from astrofd_amd.core import boundary, nozzle, rmhd_core, source_terms, srhd_core
+from astrofd_amd.settings import finalize, load_config_file
+from tests.gpu_helpers import log, rel_err
+from tests.golden_states import smooth_state, srhd_state
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _settings(**kw):
+    base = dict(NX=12, NY=10, NZ=9, NG=3, CFL=0.4, P_MAX=2.0, V_MAX=0.9999, LIMITER="mc", RECON="weno",
+                RIEMANN="hlle", RK_ORDER=2, JET_RADIUS=0.3, GAMMA_JET=2.0, ETA_RHO=0.1, P_EQ=0.01,
+                SHEAR_THICK=0.0, NOZZLE_TURB=False)
+    base.update(kw)
+    return finalize(base)
+
+
+def _oracle_params(s, pr):
+    rm = s["PHYSICS"] == "rmhd"
+    return O.make_params("rmhd" if rm else "hydro", pr.shape[1:], nv=pr.shape[0], ng=s["NG"],
+                         gamma=s["GAMMA"] if rm else None, p_max=s["P_MAX"], v_max=s["V_MAX"],
+                         limiter=s["LIMITER"], recon=s["RECON"], riemann=s["RIEMANN"], rk_order=s["RK_ORDER"])
+
+
+def test_srhd_core_calls_vs_oracle():
+    log("test_srhd_core_calls_vs_oracle")
+    s = _settings(PHYSICS="hydro", RECON="muscl", LIMITER="vanleer", RIEMANN="hllc", RK_ORDER=3, NG=2)
+    srhd_core.configure(s)
+    pr = srhd_state((16, 14, 13), 31)
+    dx = dy = dz = 0.05
+    P = _oracle_params(s, pr)
+    rhs = srhd_core.compute_rhs_muscl(pr, *pr.shape[1:], dx, dy, dz)
+    assert rel_err(rhs, O.rhs(P, pr, dx, dy, dz)) <= 1e-12
+    a = srhd_core.max_char_speed(pr, *pr.shape[1:])
+    assert abs(a - O.max_char_speed(P, pr)) <= 1e-15
+    before = pr.copy()
+    out = srhd_core.step_ssprk3(pr, dx, dy, dz, 0.01)
+    assert np.array_equal(pr, before)  # functional: the input is not modified
+    ref = O.step(O.make_params("hydro", pr.shape[1:], nv=5, ng=2, p_max=2.0, v_max=0.9999, limiter="vanleer",
+                               recon="muscl", riemann="hllc", rk_order=3), pr, dx, dy, dz, 0.01)
+    log(f"  step rel err {rel_err(out, ref):.2e}")
+    assert rel_err(out, ref) <= 1e-8
+    D, Sx, Sy, Sz, tau = srhd_core.prim_to_cons(1.0, 0.5, 0.1, 0.0, 0.1)
+    ref = O.srhd_p2c(P, np.array([[1.0, 0.5, 0.1, 0.0, 0.1]]))[0]
+    assert np.allclose([D, Sx, Sy, Sz, tau], ref, rtol=1e-14, atol=0)
+    q = srhd_core.cons_to_prim(np.array([D, D]), Sx, Sy, Sz, tau)
+    assert q[0].shape == (2,) and np.allclose(q[1], 0.5, rtol=1e-9)
+
+
+def test_rmhd_core_calls_vs_oracle():
+    log("test_rmhd_core_calls_vs_oracle")
+    s = _settings(PHYSICS="rmhd", GAMMA=4.0 / 3.0, RECON="weno", RIEMANN="hlld_full", RK_ORDER=3)
+    rmhd_core.configure(s)
+    pr = smooth_state((18, 16, 15), 32, "rmhd")
+    dx = dy = dz = 0.05
+    P = _oracle_params(s, pr)
+    rhs = rmhd_core.compute_rhs_weno(pr, *pr.shape[1:], dx, dy, dz)
+    assert rel_err(rhs, O.rhs(P, pr, dx, dy, dz)) <= 1e-11
+    out = rmhd_core.step_ssprk3(pr, dx, dy, dz, 0.005)
+    ref = O.step(P, pr, dx, dy, dz, 0.005)
+    log(f"  step rel err {rel_err(out, ref):.2e}")
+    assert rel_err(out, ref) <= 1e-8
+    U = rmhd_core.prim_to_cons_rmhd(1.0, 0.3, 0.0, 0.1, 0.05, 0.1, 0.2, 0.0, 0.0)
+    q = rmhd_core.cons_to_prim_rmhd(*U)
+    assert len(q) == 9 and abs(q[0] - 1.0) < 1e-8  # v.B != 0: the reference's p2c/c2p differ slightly
+
+
+def test_boundary_and_nozzle_in_place_vs_oracle():
+    log("test_boundary_and_nozzle_in_place_vs_oracle")
+    s = _settings(PHYSICS="hydro", NG=2, NOZZLE_TURB=True, TURB_VAMP=0.05, TURB_PAMP=0.02, SHEAR_THICK=0.02)
+    srhd_core.configure(s)
+    pr = srhd_state((16, 14, 13), 33)
+    ref = pr.copy()
+    P = _oracle_params(s, pr)
+    boundary.apply_periodic_yz(pr, 2)
+    O.apply_periodic_yz(P, ref)
+    assert np.array_equal(pr, ref)
+    boundary.apply_outflow_right_x(pr, 2)
+    O.apply_outflow_right_x(P, ref)
+    assert np.array_equal(pr, ref)
+    ny, nz = pr.shape[2] - 4, pr.shape[3] - 4
+    dx, dy, dz = 1.0 / 12, 1.0 / ny, 1.0 / nz
+    rng = np.random.default_rng(5)
+    nozzle.apply_nozzle_left_x(pr, dx, dy, dz, ny, nz, 0.5, 0.5, rng, s)
+    draws = np.random.default_rng(5).random(3 * 2 * ny * nz)
+    O.apply_nozzle_left_x(P, ref, dx, dy, dz, ny, nz, 0.5, 0.5, s, draws)
+    assert np.array_equal(pr, ref)
+
+
+def test_two_temperature_in_place_vs_oracle():
+    log("test_two_temperature_in_place_vs_oracle")
+    s = _settings(PHYSICS="hydro", NG=2, TWO_TEMPERATURE=True, TEI_TAU=0.1)
+    srhd_core.configure(s)
+    pr = np.concatenate([srhd_state((12, 11, 10), 34), np.full((2, 12, 11, 10), 0.02)])
+    ref = pr.copy()
+    source_terms.apply_two_temperature(pr, 0.01, s)
+    O.apply_two_temperature(O.make_params("hydro", pr.shape[1:], nv=7, ng=2, n_passive=2), ref, 0.01, 0.1, 5)
+    assert rel_err(pr, ref) <= 1e-15
+
+
+def test_driver_matches_fixture(tmp_path, monkeypatch):
+    """python -m astrofd_amd.driver on the 32^3 SRHD jet config (periodic y/z, as the fixture)
+    reproduces the 20-step fixture of the reference loop."""
+    log("test_driver_matches_fixture")
+    from astrofd_amd import driver
+    f = np.load(os.path.join(ROOT, "tests", "golden", "driver_srhd_32.npz"))
+    s = finalize(dict(load_config_file(os.path.join(ROOT, "astrofd_amd", "config", "srhd_jet_32.json")),
+                      BC_Y="periodic", BC_Z="periodic", OUT_EVERY=10, PRINT_EVERY=10, CHECK_NAN_EVERY=5))
+    monkeypatch.chdir(tmp_path)
+    lines = []
+    out, t, steps = driver.run(s, lines.append)
+    assert steps == 20 and abs(t - f["dts"].sum()) <= 1e-12 * t
+    err = rel_err(out, f["out"])
+    log(f"  state rel err {err:.2e}; {lines[-3:]}")
+    assert err <= 1e-8
+    written = sorted(os.listdir(os.path.join(tmp_path, "results", os.listdir(tmp_path / "results")[0])))
+    assert written == ["jet3d_rank0000_step000010.npz", "jet3d_rank0000_step000020.npz"]
+    assert any(l.startswith("[rank0] t=") for l in lines) and lines[-1] == "Done."
