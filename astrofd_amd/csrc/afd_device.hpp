@@ -35,6 +35,13 @@ template <typename T> struct Phys {
 };
 
 template <typename T> AFD_D T pymax(T a, T b) { return (b > a) ? b : a; }
+// Division and square root in the compute type: IEEE in FP64 (the plain path rounds as the
+// reference); in the FP32 path the hardware reciprocal and square root (about 1 ulp), whose
+// error is far below the FP32 rounding the mixed path already accepts.
+template <typename T> AFD_D T fdiv(T a, T b) { return a / b; }
+template <> AFD_D float fdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <typename T> AFD_D T fsqrt(T x) { return sqrt(x); }
+template <> AFD_D float fsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 template <typename T> AFD_D T pymin(T a, T b) { return (b < a) ? b : a; }
 template <typename T> AFD_D T sq(T x) { return x * x; }
 
@@ -50,7 +57,7 @@ template <typename T> AFD_D T limiter(int lid, T dqL, T dqR) {
         if (dqL * dqR <= T(0)) return T(0);
         T denom = dqL + dqR;
         if (denom == T(0)) return T(0);
-        return (T(2) * dqL * dqR) / denom;
+        return fdiv(T(2) * dqL * dqR, denom);
     }
     T mm = minmod(dqL, dqR);
     return minmod(T(0.5) * (dqL + dqR), T(2) * mm);
@@ -80,11 +87,11 @@ template <typename T> AFD_D T weno5_left(T qm2, T qm1, T q0, T qp1, T qp2) {
     T IS0 = T(13.0 / 12.0) * sq(qm2 - T(2) * qm1 + q0) + T(0.25) * sq(qm2 - T(4) * qm1 + T(3) * q0);
     T IS1 = T(13.0 / 12.0) * sq(qm1 - T(2) * q0 + qp1) + T(0.25) * sq(qm1 - qp1);
     T IS2 = T(13.0 / 12.0) * sq(q0 - T(2) * qp1 + qp2) + T(0.25) * sq(T(3) * q0 - T(4) * qp1 + qp2);
-    T a0 = T(0.1) / sq(eps + IS0);
-    T a1 = T(0.6) / sq(eps + IS1);
-    T a2 = T(0.3) / sq(eps + IS2);
+    T a0 = fdiv(T(0.1), sq(eps + IS0));
+    T a1 = fdiv(T(0.6), sq(eps + IS1));
+    T a2 = fdiv(T(0.3), sq(eps + IS2));
     T s = a0 + a1 + a2;
-    T w0 = a0 / s, w1 = a1 / s, w2 = a2 / s;
+    T w0 = fdiv(a0, s), w1 = fdiv(a1, s), w2 = fdiv(a2, s);
     T p0 = T(1.0 / 3.0) * qm2 - T(7.0 / 6.0) * qm1 + T(11.0 / 6.0) * q0;
     T p1 = T(-1.0 / 6.0) * qm1 + T(5.0 / 6.0) * q0 + T(1.0 / 3.0) * qp1;
     T p2 = T(1.0 / 3.0) * q0 + T(5.0 / 6.0) * qp1 - T(1.0 / 6.0) * qp2;
@@ -99,7 +106,7 @@ template <typename T> AFD_D T weno5_right(T qp2, T qp1, T q0, T qm1, T qm2) {
 template <typename T> AFD_D T hlle_scalar(T qL, T qR, T fL, T fR, T sL, T sR) {
     if (sL >= T(0)) return fL;
     if (sR <= T(0)) return fR;
-    return (sR * fL - sL * fR + sL * sR * (qR - qL)) / (sR - sL + Lim<T>::small);
+    return fdiv(sR * fL - sL * fR + sL * sR * (qR - qL), sR - sL + Lim<T>::small);
 }
 template <int N, typename T>
 AFD_D void hlle_vec(const T *UL, const T *UR, const T *FL, const T *FR, T sL, T sR, T *out) {
@@ -113,7 +120,7 @@ AFD_D void hlle_vec(const T *UL, const T *UR, const T *FL, const T *FR, T sL, T 
         T inv = sR - sL + Lim<T>::small;
         T sLsR = sL * sR;
 #pragma unroll
-        for (int m = 0; m < N; ++m) out[m] = (sR * FL[m] - sL * FR[m] + sLsR * (UR[m] - UL[m])) / inv;
+        for (int m = 0; m < N; ++m) out[m] = fdiv(sR * FL[m] - sL * FR[m] + sLsR * (UR[m] - UL[m]), inv);
     }
 }
 
@@ -123,8 +130,8 @@ template <typename T> AFD_D void eig_speeds(T vn, T cs, T &lm, T &lp) {
     T dm = T(1) - vn * cs;
     if (dp == T(0)) dp = Lim<T>::small;
     if (dm == T(0)) dm = Lim<T>::small;
-    T p = (vn + cs) / dp;
-    T m = (vn - cs) / dm;
+    T p = fdiv(vn + cs, dp);
+    T m = fdiv(vn - cs, dm);
     p = p > T(1) ? T(1) : (p < T(-1) ? T(-1) : p);
     m = m > T(1) ? T(1) : (m < T(-1) ? T(-1) : m);
     lm = m;
@@ -147,19 +154,19 @@ AFD_D void hllc5(const T *UL, const T *UR, const T *FL, const T *FR, T sL, T sR,
     }
     T DL = UL[0], DR = UR[0];
     T denom = (DL * (sL - vL) - DR * (sR - vR)) + Lim<T>::small;
-    T sM = (pR - pL + UL[NM] * (sL - vL) - UR[NM] * (sR - vR)) / denom;
+    T sM = fdiv(pR - pL + UL[NM] * (sL - vL) - UR[NM] * (sR - vR), denom);
     T pStar = T(0.5) * (pL + pR + DL * (sL - vL) * (sM - vL) + DR * (sR - vR) * (sM - vR));
     const bool left = sM >= T(0);
     const T *U = left ? UL : UR;
     const T *F = left ? FL : FR;
     T s = left ? sL : sR, p = left ? pL : pR, v = left ? vL : vR;
     T den = s - sM + Lim<T>::small;
-    T fac = (s - v) / den;
+    T fac = fdiv(s - v, den);
     T Ust[5];
 #pragma unroll
     for (int m = 0; m < 5; ++m) Ust[m] = U[m] * fac;
-    Ust[NM] = U[NM] * fac + (pStar - p) / den;
-    Ust[4] = U[4] * fac + (pStar * sM - p * v) / den;
+    Ust[NM] = U[NM] * fac + fdiv(pStar - p, den);
+    Ust[4] = U[4] * fac + fdiv(pStar * sM - p * v, den);
 #pragma unroll
     for (int m = 0; m < 5; ++m) out[m] = F[m] + s * (Ust[m] - U[m]);
 }
@@ -189,7 +196,7 @@ AFD_D void srhd_floor(double p_max, double v_max, double *q) {
 // core/srhd_core.py:46-59 with the Lorentz factor W supplied
 template <typename T> AFD_D void srhd_p2c_W(const Phys<T> &ph, const T *q, T W, T *U) {
     T rho = q[0], p = q[4];
-    T h = T(1) + ph.gfac * p / pymax(rho, Lim<T>::small);
+    T h = T(1) + fdiv(ph.gfac * p, pymax(rho, Lim<T>::small));
     T w = rho * h;
     T D = rho * W;
     T wWW = w * W * W;
@@ -213,12 +220,12 @@ template <int DIR, typename T> AFD_D void srhd_flux(const T *q, const T *U, T *F
 
 // core/srhd_core.py:133-140
 template <typename T> AFD_D T srhd_sound_speed(const Phys<T> &ph, T rho, T p) {
-    T h = T(1) + ph.gfac * p / pymax(rho, Lim<T>::small);
+    T h = T(1) + fdiv(ph.gfac * p, pymax(rho, Lim<T>::small));
     T w = rho * h;
-    T cs2 = ph.gamma * p / pymax(w, Lim<T>::small);
+    T cs2 = fdiv(ph.gamma * p, pymax(w, Lim<T>::small));
     if (cs2 < T(0)) cs2 = T(0);
     if (cs2 > Lim<T>::one_minus) cs2 = Lim<T>::one_minus;
-    return sqrt(cs2);
+    return fsqrt(cs2);
 }
 
 // ---------------------------------------------------------------- RMHD
@@ -249,13 +256,13 @@ template <typename T> AFD_D void rmhd_p2c_W(const Phys<T> &ph, const T *q, const
     if (p < Lim<T>::small) p = Lim<T>::small;
     if (p > ph.p_max) p = ph.p_max;
     T B2 = Bx * Bx + By * By + Bz * Bz;
-    T h = T(1) + ph.gfac * p / pymax(rho, Lim<T>::small);
+    T h = T(1) + fdiv(ph.gfac * p, pymax(rho, Lim<T>::small));
     T rhoh = rho * h;
     T vb = vx * Bx + vy * By + vz * Bz;
     T b0 = W * vb;
-    T bx = (Bx + b0 * vx) / W;
-    T by = (By + b0 * vy) / W;
-    T bz = (Bz + b0 * vz) / W;
+    T bx = fdiv(Bx + b0 * vx, W);
+    T by = fdiv(By + b0 * vy, W);
+    T bz = fdiv(Bz + b0 * vz, W);
     T b2 = (bx * bx + by * by + bz * bz) - b0 * b0;
     T rhohWW = rhoh * W * W;
     T wtot = rhohWW + B2;
@@ -300,27 +307,27 @@ template <typename T> AFD_D T rmhd_fast_speed_W(const Phys<T> &ph, const T *q, T
     if (rho < Lim<T>::small) rho = Lim<T>::small;
     if (p < Lim<T>::small) p = Lim<T>::small;
     if (p > ph.p_max) p = ph.p_max;
-    T h = T(1) + ph.gfac * p / pymax(rho, Lim<T>::small);
+    T h = T(1) + fdiv(ph.gfac * p, pymax(rho, Lim<T>::small));
     T rhoh = rho * h;
     T vb = vx * Bx + vy * By + vz * Bz;
     T b0 = W * vb;
-    T bx = (Bx + b0 * vx) / W;
-    T by = (By + b0 * vy) / W;
-    T bz = (Bz + b0 * vz) / W;
+    T bx = fdiv(Bx + b0 * vx, W);
+    T by = fdiv(By + b0 * vy, W);
+    T bz = fdiv(Bz + b0 * vz, W);
     T b2 = (bx * bx + by * by + bz * bz) - b0 * b0;
-    T cs2 = ph.gamma * p / pymax(rhoh, Lim<T>::small);
+    T cs2 = fdiv(ph.gamma * p, pymax(rhoh, Lim<T>::small));
     if (cs2 < T(0)) cs2 = T(0);
     if (cs2 > Lim<T>::one_minus) cs2 = Lim<T>::one_minus;
-    T ca2 = b2 / pymax(rhoh + b2, Lim<T>::small);
+    T ca2 = fdiv(b2, pymax(rhoh + b2, Lim<T>::small));
     if (ca2 < T(0)) ca2 = T(0);
     if (ca2 > Lim<T>::one_minus) ca2 = Lim<T>::one_minus;
     T a = cs2 + ca2 - cs2 * ca2;
     T disc = a * a - T(4) * cs2 * ca2;
     if (disc < T(0)) disc = T(0);
-    T cf2 = T(0.5) * (a + sqrt(disc));
+    T cf2 = T(0.5) * (a + fsqrt(disc));
     if (cf2 < T(0)) cf2 = T(0);
     if (cf2 > Lim<T>::one_minus) cf2 = Lim<T>::one_minus;
-    return sqrt(cf2);
+    return fsqrt(cf2);
 }
 
 // W used by rmhd_fast_speed: v2 capped at V_MAX^2 - 1e-14 (core/rmhd_core.py:422-425)
@@ -334,7 +341,7 @@ AFD_D double rmhd_fast_W(double v_max, const double *q) {
 template <int NM, typename T>
 AFD_D T hllc_contact_speed(const T *UL, const T *UR, T sL, T sR, T pL, T pR, T vL, T vR) {
     T denom = (UL[0] * (sL - vL) - UR[0] * (sR - vR)) + Lim<T>::small;
-    return (pR - pL + UL[NM] * (sL - vL) - UR[NM] * (sR - vR)) / denom;
+    return fdiv(pR - pL + UL[NM] * (sL - vL) - UR[NM] * (sR - vR), denom);
 }
 
 // p2c of a star state built from T-valued primitives: the Lorentz factor in FP64
@@ -371,8 +378,8 @@ AFD_D void hlld_full(const Phys<T> &ph, const T *UL, const T *UR, const T *FL, c
     T sM = T(0), DstarL = T(0), DstarR = T(0);
     if (!fallback) {
         sM = hllc_contact_speed<1 + DIR>(UL, UR, sL, sR, pL, pR, vL, vR);
-        DstarL = UL[0] * (sL - vL) / (sL - sM + small);
-        DstarR = UR[0] * (sR - vR) / (sR - sM + small);
+        DstarL = fdiv(UL[0] * (sL - vL), sL - sM + small);
+        DstarR = fdiv(UR[0] * (sR - vR), sR - sM + small);
         fallback = (DstarL <= small || DstarR <= small);
     }
     T denomL = UL[0] * (sL - vL) - Bn * Bn;
@@ -382,23 +389,20 @@ AFD_D void hlld_full(const Phys<T> &ph, const T *UL, const T *UR, const T *FL, c
         hlle_vec<9>(UL, UR, FL, FR, sL, sR, out);
         return;
     }
-    T facL = (sL - vL) / (sL - sM + small), facR = (sR - vR) / (sR - sM + small);
-    T Bt1L_s = UL[bt1] * (sL - vL) / (sL - sM + small);
-    T Bt2L_s = UL[bt2] * (sL - vL) / (sL - sM + small);
-    T Bt1R_s = UR[bt1] * (sR - vR) / (sR - sM + small);
-    T Bt2R_s = UR[bt2] * (sR - vR) / (sR - sM + small);
-    (void)facL;
-    (void)facR;
-    T vt1L = UL[vt1] / pymax(UL[0], small);
-    T vt2L = UL[vt2] / pymax(UL[0], small);
-    T vt1R = UR[vt1] / pymax(UR[0], small);
-    T vt2R = UR[vt2] / pymax(UR[0], small);
-    T vt1L_s = vt1L - Bn * (UL[bt1] - Bt1L_s) / denomL;
-    T vt2L_s = vt2L - Bn * (UL[bt2] - Bt2L_s) / denomL;
-    T vt1R_s = vt1R - Bn * (UR[bt1] - Bt1R_s) / denomR;
-    T vt2R_s = vt2R - Bn * (UR[bt2] - Bt2R_s) / denomR;
-    T sL_star = sM - fabs(Bn) / sqrt(DstarL + small);
-    T sR_star = sM + fabs(Bn) / sqrt(DstarR + small);
+    T Bt1L_s = fdiv(UL[bt1] * (sL - vL), sL - sM + small);
+    T Bt2L_s = fdiv(UL[bt2] * (sL - vL), sL - sM + small);
+    T Bt1R_s = fdiv(UR[bt1] * (sR - vR), sR - sM + small);
+    T Bt2R_s = fdiv(UR[bt2] * (sR - vR), sR - sM + small);
+    T vt1L = fdiv(UL[vt1], pymax(UL[0], small));
+    T vt2L = fdiv(UL[vt2], pymax(UL[0], small));
+    T vt1R = fdiv(UR[vt1], pymax(UR[0], small));
+    T vt2R = fdiv(UR[vt2], pymax(UR[0], small));
+    T vt1L_s = vt1L - fdiv(Bn * (UL[bt1] - Bt1L_s), denomL);
+    T vt2L_s = vt2L - fdiv(Bn * (UL[bt2] - Bt2L_s), denomL);
+    T vt1R_s = vt1R - fdiv(Bn * (UR[bt1] - Bt1R_s), denomR);
+    T vt2R_s = vt2R - fdiv(Bn * (UR[bt2] - Bt2R_s), denomR);
+    T sL_star = sM - fdiv(fabs(Bn), fsqrt(DstarL + small));
+    T sR_star = sM + fdiv(fabs(Bn), fsqrt(DstarR + small));
     // region of the fan containing x/t = 0 (core/rmhd_core.py:716-724)
     int region;
     if (sL <= T(0) && T(0) <= sL_star) region = 1;
@@ -433,13 +437,13 @@ AFD_D void hlld_full(const Phys<T> &ph, const T *UL, const T *UR, const T *FL, c
         return;
     }
     // double-star state
-    T sqrtDL = sqrt(DstarL), sqrtDR = sqrt(DstarR);
+    T sqrtDL = fsqrt(DstarL), sqrtDR = fsqrt(DstarR);
     T sgn = (Bn >= T(0)) ? T(1) : T(-1);
     T den = sqrtDL + sqrtDR + small;
-    T vt1_ss = (sqrtDL * vt1L_s + sqrtDR * vt1R_s + sgn * (Bt1R_s - Bt1L_s)) / den;
-    T vt2_ss = (sqrtDL * vt2L_s + sqrtDR * vt2R_s + sgn * (Bt2R_s - Bt2L_s)) / den;
-    T Bt1_ss = (sqrtDL * Bt1R_s + sqrtDR * Bt1L_s + sgn * sqrtDL * sqrtDR * (vt1L_s - vt1R_s)) / den;
-    T Bt2_ss = (sqrtDL * Bt2R_s + sqrtDR * Bt2L_s + sgn * sqrtDL * sqrtDR * (vt2L_s - vt2R_s)) / den;
+    T vt1_ss = fdiv(sqrtDL * vt1L_s + sqrtDR * vt1R_s + sgn * (Bt1R_s - Bt1L_s), den);
+    T vt2_ss = fdiv(sqrtDL * vt2L_s + sqrtDR * vt2R_s + sgn * (Bt2R_s - Bt2L_s), den);
+    T Bt1_ss = fdiv(sqrtDL * Bt1R_s + sqrtDR * Bt1L_s + sgn * sqrtDL * sqrtDR * (vt1L_s - vt1R_s), den);
+    T Bt2_ss = fdiv(sqrtDL * Bt2R_s + sqrtDR * Bt2L_s + sgn * sqrtDL * sqrtDR * (vt2L_s - vt2R_s), den);
     T Dss = T(0.5) * (DstarL + DstarR);
     q[0] = pymax(Dss, small);
     q[1 + DIR] = sM;

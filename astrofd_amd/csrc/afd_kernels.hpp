@@ -26,6 +26,18 @@ struct Grid {
     long sx, sy, ncell;   // strides of x and y, number of cells
 };
 
+// Minimum waves per SIMD the face kernels are compiled for (the register budget follows:
+// 3 waves <= 168 VGPRs, 4 waves <= 128 VGPRs).
+#ifndef AFD_FACE_WAVES_F64
+#define AFD_FACE_WAVES_F64 3
+#endif
+#ifndef AFD_FACE_WAVES_F32
+#define AFD_FACE_WAVES_F32 4
+#endif
+template <typename T> struct FaceOcc {
+    static constexpr int waves = sizeof(T) == 8 ? AFD_FACE_WAVES_F64 : AFD_FACE_WAVES_F32;
+};
+
 inline dim3 grid1d(long n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
 
 AFD_D long cidx(const Grid &g, int i, int j, int k) { return ((long)i * g.ny + j) * g.nz + k; }
@@ -208,7 +220,8 @@ AFD_D void rmhd_riemann(const Phys<T> &ph, const RmhdFace<DIR, T> &f, T sL, T sR
 }
 
 template <int RECON, int RIEM, int DIR, typename T>
-__global__ __launch_bounds__(BLOCK) void k_face_rmhd(FaceArgs<T> a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(FaceOcc<T>::waves)))
+void k_face_rmhd(FaceArgs<T> a) {
     const Grid &g = a.g;
     int i, j, k;
     if (!face_coords<DIR>(g, (long)blockIdx.x * BLOCK + threadIdx.x, i, j, k)) return;

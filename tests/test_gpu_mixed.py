@@ -1,0 +1,60 @@
+"""GPU tests of the mixed-precision path (FP32 reconstruction and Riemann fluxes; FP64 face
+states, Lorentz factor, con2prim and SSPRK accumulation) against the FP64 CPU oracle.
+
+Bar: the metric's L1(rho) <= 1e-6 relative after a step, for rho, p and W, on smooth seeded
+states, and max |error| within FP32 resolution of the flux update (dt / dx * |F| * 2^-24
+times a modest growth factor, <= 1e-4 relative here)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.gpu_helpers import log, make_domain
+from tests.golden_states import smooth_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _lorentz(pr):
+    v2 = pr[1] ** 2 + pr[2] ** 2 + pr[3] ** 2
+    return 1.0 / np.sqrt(1.0 - np.minimum(v2, 1.0 - 1e-14))
+
+
+CASES = [
+    ("hydro", "muscl", "vanleer", "hllc", 3),
+    ("hydro", "weno", "mc", "hlle", 2),
+    ("hydro", "ppm", "mc", "hllc", 3),
+    ("rmhd", "weno", "mc", "hlld_full", 3),
+    ("rmhd", "ppm", "mc", "hlld", 3),
+    ("rmhd", "muscl", "minmod", "hlle", 2),
+]
+
+
+@pytest.mark.parametrize("phys,recon,lim,riem,rk", CASES)
+def test_mixed_step_vs_oracle(phys, recon, lim, riem, rk):
+    log(f"test_mixed_step_vs_oracle {phys} {recon} {lim} {riem} rk{rk}")
+    ng = 3
+    shape = (22 + 2 * ng, 20 + 2 * ng, 18 + 2 * ng)
+    pr = smooth_state(shape, 91, phys, vamp=0.5)
+    gamma = 4.0 / 3.0 if phys == "rmhd" else None
+    dx = 1.0 / 22
+    dt = 0.4 * dx * 0.5
+    d = make_domain(phys, pr.shape, ng, dxyz=(dx, dx, dx), gamma=gamma, recon=recon, limiter=lim, riemann=riem,
+                    rk_order=rk, precision="mixed")
+    d.set_prim(pr)
+    d.step(dt)
+    out = d.get_prim()
+    d.close()
+    P = O.make_params(phys, pr.shape[1:], nv=pr.shape[0], ng=ng, gamma=gamma, p_max=2.0, v_max=0.9999,
+                      limiter=lim, recon=recon, riemann=riem, rk_order=rk)
+    ref = O.step(P, pr, dx, dx, dx, dt)
+    core = (slice(ng, -ng),) * 3
+    for name, a, b in (("rho", out[0], ref[0]), ("p", out[4], ref[4]), ("W", _lorentz(out), _lorentz(ref))):
+        a, b = a[core], b[core]
+        l1 = np.abs(a - b).sum() / np.abs(b).sum()
+        mx = np.abs(a - b).max() / np.abs(b).max()
+        log(f"  {name}: L1 rel {l1:.2e}  max rel {mx:.2e}")
+        assert l1 <= 1e-6, name
+        assert mx <= 1e-4, name
+    # the FP64 parts keep B and psi close as well
+    for v in range(5, pr.shape[0]):
+        assert np.abs(out[v][core] - ref[v][core]).max() <= 1e-4 * max(np.abs(ref[v][core]).max(), 1e-3)
