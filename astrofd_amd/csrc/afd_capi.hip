@@ -63,6 +63,9 @@ struct afd_ctx {
     std::vector<hipEvent_t> ev_pool;
     // learned guess
     MlpDevice mlp{};
+    int *c2p_queue = nullptr;         // cells whose learned start the judge rejected (QUEUE_SLOTS regions)
+    unsigned *c2p_qcount = nullptr;
+    long c2p_qcap = 0;
     std::vector<void *> allocs;
 };
 
@@ -248,6 +251,12 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
     TRY(dalloc(c, (void **)&c->cnt_out, 8));
     TRY(dalloc(c, (void **)&c->stats_dev, sizeof(unsigned long long) * STATS_N * STATS_SLOTS));
     if (nb == 9) TRY(mlp_alloc(c->mlp, [c](void **ptr, size_t bytes) { return dalloc(c, ptr, bytes); }));
+    if (nb == 9 && p.c2p_guess == AFD_C2P_LEARNED) {
+        long blocks = (long)grid1d(c->g.ncell).x;
+        c->c2p_qcap = (blocks + QUEUE_SLOTS - 1) / QUEUE_SLOTS * BLOCK;
+        TRY(dalloc(c, (void **)&c->c2p_queue, sizeof(int) * (size_t)c->c2p_qcap * QUEUE_SLOTS));
+        TRY(dalloc(c, (void **)&c->c2p_qcount, sizeof(unsigned) * QUEUE_SLOTS));
+    }
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         afd_destroy(c);
@@ -575,6 +584,9 @@ static UpdLaunch upd_launch(afd_ctx *c, const double *Pin, const double *P0, dou
     a.p_max = c->p.p_max;
     a.v_max = c->p.v_max;
     a.mlp = c->p.c2p_guess == AFD_C2P_LEARNED ? c->mlp.w : nullptr;
+    a.queue = a.mlp ? c->c2p_queue : nullptr;
+    a.qcount = a.mlp ? c->c2p_qcount : nullptr;
+    a.qcap = c->c2p_qcap;
     return l;
 }
 

@@ -269,6 +269,33 @@ hipError_t launch_face_rmhd(const FaceLaunch &l, hipStream_t s) {
     return hipGetLastError();
 }
 
+// The stage update runs fused (update + con2prim in one kernel) or split (update, then
+// k_c2p_cells over Uc); AFD_SPLIT_C2P selects the split form at build time.
+#ifndef AFD_SPLIT_C2P
+#define AFD_SPLIT_C2P 0
+#endif
+template <int NB, typename F, bool LEARNED> static void upd_stage(const UpdArgs<F> &a, dim3 grid, hipStream_t s) {
+    if constexpr (LEARNED) {
+        if (a.queue) {  // rejected learned starts are queued and solved densely afterwards
+            (void)hipMemsetAsync(a.qcount, 0, sizeof(unsigned) * QUEUE_SLOTS, s);
+            if constexpr (AFD_SPLIT_C2P) {
+                hipLaunchKernelGGL((k_update<NB, F, LEARNED, false>), grid, dim3(BLOCK), 0, s, a);
+                hipLaunchKernelGGL((k_c2p_cells<NB, F, LEARNED>), grid, dim3(BLOCK), 0, s, a);
+            } else {
+                hipLaunchKernelGGL((k_update<NB, F, LEARNED, true>), grid, dim3(BLOCK), 0, s, a);
+            }
+            hipLaunchKernelGGL((k_c2p_queue<F>), dim3(16, QUEUE_SLOTS), dim3(BLOCK), 0, s, a);
+            return;
+        }
+    }
+    if constexpr (AFD_SPLIT_C2P) {
+        hipLaunchKernelGGL((k_update<NB, F, LEARNED, false>), grid, dim3(BLOCK), 0, s, a);
+        hipLaunchKernelGGL((k_c2p_cells<NB, F, LEARNED>), grid, dim3(BLOCK), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((k_update<NB, F, LEARNED, true>), grid, dim3(BLOCK), 0, s, a);
+    }
+}
+
 template <int NB, typename F> static void upd(const UpdLaunch &l, bool rhs_only, hipStream_t s) {
     UpdArgs<F> a;
     static_assert(sizeof(UpdArgs<F>) == sizeof(UpdArgs<double>), "layout");
@@ -276,10 +303,10 @@ template <int NB, typename F> static void upd(const UpdLaunch &l, bool rhs_only,
     dim3 grid = grid1d(l.a.g.ncell);
     if (rhs_only) hipLaunchKernelGGL((k_rhs<NB, F>), grid, dim3(BLOCK), 0, s, a);
     else if constexpr (NB == 9) {
-        if (l.a.mlp) hipLaunchKernelGGL((k_update<NB, F, true>), grid, dim3(BLOCK), 0, s, a);
-        else hipLaunchKernelGGL((k_update<NB, F, false>), grid, dim3(BLOCK), 0, s, a);
+        if (l.a.mlp) upd_stage<NB, F, true>(a, grid, s);
+        else upd_stage<NB, F, false>(a, grid, s);
     } else {
-        hipLaunchKernelGGL((k_update<NB, F, false>), grid, dim3(BLOCK), 0, s, a);
+        upd_stage<NB, F, false>(a, grid, s);
     }
 }
 

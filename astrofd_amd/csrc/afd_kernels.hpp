@@ -27,12 +27,13 @@ struct Grid {
 };
 
 // Minimum waves per SIMD the face kernels are compiled for (the register budget follows:
-// 3 waves <= 168 VGPRs, 4 waves <= 128 VGPRs).
+// 3 waves <= 168 VGPRs).  Measured on the 256^3 RMHD WENO5 + HLLD step: FP64 faces gain ~3 %
+// at 3 waves over the unconstrained 2; FP32 faces lose ~20 % at 4 (spills) against 3.
 #ifndef AFD_FACE_WAVES_F64
 #define AFD_FACE_WAVES_F64 3
 #endif
 #ifndef AFD_FACE_WAVES_F32
-#define AFD_FACE_WAVES_F32 4
+#define AFD_FACE_WAVES_F32 3
 #endif
 template <typename T> struct FaceOcc {
     static constexpr int waves = sizeof(T) == 8 ? AFD_FACE_WAVES_F64 : AFD_FACE_WAVES_F32;
@@ -121,8 +122,8 @@ __global__ __launch_bounds__(BLOCK) void k_face_srhd(FaceArgs<T> a) {
     for (int v = 0; v < 5; ++v) recon_face<RECON, T>(a.P + v * g.ncell, c, s, a.lim, qLd[v], qRd[v]);
     srhd_floor(a.p_max, a.v_max, qLd);
     srhd_floor(a.p_max, a.v_max, qRd);
-    T WL = (T)lorentz_guarded(qLd[1] * qLd[1] + qLd[2] * qLd[2] + qLd[3] * qLd[3]);
-    T WR = (T)lorentz_guarded(qRd[1] * qRd[1] + qRd[2] * qRd[2] + qRd[3] * qRd[3]);
+    T WL = (T)lorentz_guarded<sizeof(T) == 4>(qLd[1] * qLd[1] + qLd[2] * qLd[2] + qLd[3] * qLd[3]);
+    T WR = (T)lorentz_guarded<sizeof(T) == 4>(qRd[1] * qRd[1] + qRd[2] * qRd[2] + qRd[3] * qRd[3]);
     T qL[5], qR[5], UL[5], UR[5], FL[5], FR[5], F[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
@@ -168,11 +169,12 @@ AFD_D void rmhd_face_setup(const FaceArgs<T> &a, long c, RmhdFace<DIR, T> &f) {
     rmhd_floor(a.p_max, a.v_max, qLd);
     rmhd_floor(a.p_max, a.v_max, qRd);
     double vcL[3], vcR[3];
-    double WLd = rmhd_cap_W(a.v_max, qLd, vcL), WRd = rmhd_cap_W(a.v_max, qRd, vcR);
+    constexpr bool FAST = sizeof(T) == 4;  // mixed path: FP64 Lorentz factors with the fast rsq
+    double WLd = rmhd_cap_W<FAST>(a.v_max, qLd, vcL), WRd = rmhd_cap_W<FAST>(a.v_max, qRd, vcR);
     // rmhd_fast_speed's own Lorentz factor differs only when the floored v2 still reaches V_MAX^2
     double WfL = WLd, WfR = WRd;
-    if (qLd[1] * qLd[1] + qLd[2] * qLd[2] + qLd[3] * qLd[3] >= a.v_max * a.v_max) WfL = rmhd_fast_W(a.v_max, qLd);
-    if (qRd[1] * qRd[1] + qRd[2] * qRd[2] + qRd[3] * qRd[3] >= a.v_max * a.v_max) WfR = rmhd_fast_W(a.v_max, qRd);
+    if (qLd[1] * qLd[1] + qLd[2] * qLd[2] + qLd[3] * qLd[3] >= a.v_max * a.v_max) WfL = rmhd_fast_W<FAST>(a.v_max, qLd);
+    if (qRd[1] * qRd[1] + qRd[2] * qRd[2] + qRd[3] * qRd[3] >= a.v_max * a.v_max) WfR = rmhd_fast_W<FAST>(a.v_max, qRd);
     T qL[9], qR[9], vL3[3], vR3[3];
 #pragma unroll
     for (int v = 0; v < 9; ++v) {
@@ -326,7 +328,26 @@ template <typename F> struct UpdArgs {
     double dt, dx, dy, dz, glm_cp2, resistivity;
     double gamma, gfac, p_max, v_max;
     const void *mlp;              // learned-guess weights (null: plain con2prim)
+    int *queue;                   // learned guess: cells the judge rejected, QUEUE_SLOTS regions
+    unsigned *qcount;             //   of qcap entries each, filled per workgroup slot (or null)
+    long qcap;
 };
+
+// Rejected cells of the learned guess are queued per workgroup slot (blockIdx % QUEUE_SLOTS),
+// each slot a region of qcap = ceil(blocks / QUEUE_SLOTS) * BLOCK entries, so no slot overflows
+// and few waves contend for one counter; k_c2p_queue then solves them densely.
+constexpr int QUEUE_SLOTS = 64;
+AFD_D void wave_enqueue(bool need, long c, int *queue, unsigned *qcount, long qcap) {
+    const unsigned long long m = __ballot(need);
+    if (m == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    const int slot = blockIdx.x % QUEUE_SLOTS;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(&qcount[slot], (unsigned)__popcll(m));
+    base = __shfl(base, leader);
+    if (need) queue[(long)slot * qcap + base + __popcll(m & ((1ull << lane) - 1ull))] = (int)c;
+}
 
 AFD_D double ssprk_combine(int stage, int nst, double u0, double uc, double r) {
 #pragma clang fp contract(off)
@@ -361,12 +382,23 @@ template <int NB, typename F> AFD_D void cell_rhs(const UpdArgs<F> &a, long c, b
 #pragma unroll
     for (int v = 0; v < NB; ++v) rhs[v] = 0.0;
     if (!in) return;
+    if constexpr (sizeof(F) == 8) {  // FP64 path: the reference's divisions
 #pragma unroll
-    for (int v = 0; v < NB; ++v) {
-        long o = v * g.ncell + c;
-        rhs[v] -= ((double)a.Fx[o + g.sx] - (double)a.FxL[o]) / a.dx;
-        rhs[v] -= ((double)a.Fy[o + g.sy] - (double)a.Fy[o]) / a.dy;
-        rhs[v] -= ((double)a.Fz[o + 1] - (double)a.Fz[o]) / a.dz;
+        for (int v = 0; v < NB; ++v) {
+            long o = v * g.ncell + c;
+            rhs[v] -= ((double)a.Fx[o + g.sx] - (double)a.FxL[o]) / a.dx;
+            rhs[v] -= ((double)a.Fy[o + g.sy] - (double)a.Fy[o]) / a.dy;
+            rhs[v] -= ((double)a.Fz[o + 1] - (double)a.Fz[o]) / a.dz;
+        }
+    } else {  // mixed path (FP32 fluxes): multiply by the inverse spacings
+        const double ix = 1.0 / a.dx, iy = 1.0 / a.dy, iz = 1.0 / a.dz;
+#pragma unroll
+        for (int v = 0; v < NB; ++v) {
+            long o = v * g.ncell + c;
+            rhs[v] -= ((double)a.Fx[o + g.sx] - (double)a.FxL[o]) * ix;
+            rhs[v] -= ((double)a.Fy[o + g.sy] - (double)a.Fy[o]) * iy;
+            rhs[v] -= ((double)a.Fz[o + 1] - (double)a.Fz[o]) * iz;
+        }
     }
     if (NB == 9) {
         rhs[8 % NB] -= a.glm_cp2 * a.Pin[8 * g.ncell + c];
@@ -403,9 +435,38 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(UpdArgs<F> a) {
     for (int t = 0; t < a.np; ++t) a.Pout[(NB + t) * g.ncell + c] = a.rhs_pass ? a.rhs_pass[t * g.ncell + c] : 0.0;
 }
 
-// LEARNED: the RMHD con2prim starts from the network's guess (a.mlp); a separate instantiation
-// keeps the network's registers out of the plain kernel.
-template <int NB, typename F, bool LEARNED>
+// con2prim of one cell's conserved state U (FP64).  LEARNED: the RMHD Newton solve starts from
+// the network's guess, judged and backed by the plain solve: inline, or (queue set) by queueing
+// the rejected cell for k_c2p_queue.  Reached by every lane of the wave (the network runs
+// wave-wide); `live` marks lanes with a cell.  Returns whether q holds the cell's primitives.
+template <int NB, bool LEARNED, typename F>
+AFD_D bool cell_c2p(const UpdArgs<F> &a, const double *U, bool live, long c, double *q, C2PStats &st) {
+    constexpr bool FAST = sizeof(F) == 4;  // the mixed path's con2prim uses the fast FP64 rcp/rsq
+    if constexpr (NB == 9) {
+        RmhdC2PParams cp = {a.gamma, a.gfac, a.p_max, a.v_max};
+        bool done = false;
+        if constexpr (LEARNED) {
+            float x[5], y[2];
+            bool fok = mlp_features(U, x) && live;
+            mlp_infer_wave(mlp_view(a.mlp), x, y);
+            if (fok) done = rmhd_c2p_from_guess<FAST>(cp, U, mlp_guess_Z(U[0], a.gfac, y), q, st);
+            if (a.queue) {
+                wave_enqueue(live && !done, c, a.queue, a.qcount, a.qcap);
+                return done;
+            }
+        }
+        if (live && !done) rmhd_c2p<FAST>(cp, U, q, st);
+    } else {
+        if (live) hydro_c2p<FAST>(a.gamma, a.gfac, a.p_max, a.v_max, U, q, st);
+    }
+    return live;
+}
+
+// Stage update of one cell: flux divergence + sources, SSPRK combination into Uc (FP64), the
+// passives on primitives, and (C2P) the con2prim of the new state into Pout.  Without C2P the
+// con2prim runs in k_c2p_cells on Uc.  LEARNED selects the network's Newton start; a separate
+// instantiation keeps its registers out of the plain kernel.
+template <int NB, typename F, bool LEARNED, bool C2P = true>
 __global__ __launch_bounds__(BLOCK) void k_update(UpdArgs<F> a) {
     const Grid &g = a.g;
     const long c = (long)blockIdx.x * BLOCK + threadIdx.x;
@@ -441,22 +502,13 @@ __global__ __launch_bounds__(BLOCK) void k_update(UpdArgs<F> a) {
             a.Uc[v * g.ncell + c] = U[v];
         }
     }
-    if constexpr (NB == 9) {
-        RmhdC2PParams cp = {a.gamma, a.gfac, a.p_max, a.v_max};
-        bool done = false;
-        if constexpr (LEARNED) {  // every lane of the wave runs the network
-            float x[5], y[2];
-            bool fok = mlp_features(U, x) && live;
-            mlp_infer_wave(mlp_view(a.mlp), x, y);
-            if (fok) done = rmhd_c2p_from_guess(cp, U, mlp_guess_Z(U[0], a.gfac, y), q, st);
-        }
-        if (live && !done) rmhd_c2p(cp, U, q, st);
-    } else {
-        if (live) hydro_c2p(a.gamma, a.gfac, a.p_max, a.v_max, U, q, st);
-    }
+    bool have = false;
+    if constexpr (C2P) have = cell_c2p<NB, LEARNED>(a, U, live, c, q, st);
     if (live) {
+        if (have) {
 #pragma unroll
-        for (int v = 0; v < NB; ++v) a.Pout[v * g.ncell + c] = q[v];
+            for (int v = 0; v < NB; ++v) a.Pout[v * g.ncell + c] = q[v];
+        }
         for (int t = 0; t < a.np; ++t) {
             long o = (long)(a.po + t) * g.ncell + c;
             double rt = a.rhs_pass ? a.rhs_pass[t * g.ncell + c] : 0.0;
@@ -467,6 +519,46 @@ __global__ __launch_bounds__(BLOCK) void k_update(UpdArgs<F> a) {
             }
             a.Pout[o] = ssprk_combine(a.stage, a.nst, a.P0[o], a.Pin[o], r);
         }
+    }
+    if constexpr (C2P) {
+        if (a.stats) wave_add_stats(a.stats, st);
+    }
+}
+
+// con2prim of every cell of Uc into Pout[0, NB) (the split form of the stage update).
+template <int NB, typename F, bool LEARNED>
+__global__ __launch_bounds__(BLOCK) void k_c2p_cells(UpdArgs<F> a) {
+    const long c = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const long n = a.g.ncell;
+    const bool live = c < n;
+    C2PStats st = {0, 0, 0};
+    double U[NB], q[NB];
+#pragma unroll
+    for (int v = 0; v < NB; ++v) U[v] = live ? a.Uc[v * n + c] : 0.0;
+    if (cell_c2p<NB, LEARNED>(a, U, live, c, q, st)) {
+#pragma unroll
+        for (int v = 0; v < NB; ++v) a.Pout[v * n + c] = q[v];
+    }
+    if (a.stats) wave_add_stats(a.stats, st);
+}
+
+// Plain RMHD con2prim of the queued cells (rejected learned starts), densely packed:
+// blockIdx.y = queue slot, grid-stride over the slot's entries.
+template <typename F> __global__ __launch_bounds__(BLOCK) void k_c2p_queue(UpdArgs<F> a) {
+    const int slot = blockIdx.y;
+    const long n = a.g.ncell;
+    const unsigned cnt = a.qcount[slot];
+    const int *q_slot = a.queue + (long)slot * a.qcap;
+    RmhdC2PParams cp = {a.gamma, a.gfac, a.p_max, a.v_max};
+    C2PStats st = {0, 0, 0};
+    for (long k = (long)blockIdx.x * BLOCK + threadIdx.x; k < (long)cnt; k += (long)gridDim.x * BLOCK) {
+        const long c = q_slot[k];
+        double U[9], q[9];
+#pragma unroll
+        for (int v = 0; v < 9; ++v) U[v] = a.Uc[v * n + c];
+        rmhd_c2p<sizeof(F) == 4>(cp, U, q, st);
+#pragma unroll
+        for (int v = 0; v < 9; ++v) a.Pout[v * n + c] = q[v];
     }
     if (a.stats) wave_add_stats(a.stats, st);
 }

@@ -167,15 +167,18 @@ AFD_D bool mlp_features(const double *U, float x[5]) {
         for (int f = 0; f < 5; ++f) x[f] = 0.0f;
         return false;
     }
-    const float l10 = 0.30102999566f;  // log10(2)
-    float Df = (float)D;
-    float Sm = (float)sqrt(S2);
-    x[0] = __log2f(Df) * l10 * 0.25f;
-    x[1] = __log2f((float)(tau / D)) * l10 * 0.25f;
-    x[2] = __log2f((float)(sqrt(S2) / D) + 1e-10f) * l10 * 0.25f;
-    x[3] = __log2f((float)(B2 / D) + 1e-10f) * l10 * 0.25f;
-    float den = Sm * (float)sqrt(B2);
-    x[4] = den > 0.0f ? (float)SB / den : 0.0f;
+    // FP32 from here: the features only need FP32 resolution (log2 / rcp / sqrt instructions)
+    const float c = 0.30102999566f * 0.25f;  // log10(2) / 4
+    const float Df = (float)D, S2f = (float)S2, B2f = (float)B2;
+    const float rD = __builtin_amdgcn_rcpf(Df);
+    const float lD = __log2f(Df);
+    const float Sm = __builtin_amdgcn_sqrtf(S2f), Bm = __builtin_amdgcn_sqrtf(B2f);
+    x[0] = lD * c;
+    x[1] = (__log2f((float)tau) - lD) * c;
+    x[2] = __log2f(Sm * rD + 1e-10f) * c;
+    x[3] = __log2f(B2f * rD + 1e-10f) * c;
+    const float den = Sm * Bm;
+    x[4] = den > 0.0f ? (float)SB * __builtin_amdgcn_rcpf(den) : 0.0f;
     for (int f = 0; f < 5; ++f)
         if (!isfinite(x[f])) ok = false;
     return ok;
@@ -183,8 +186,9 @@ AFD_D bool mlp_features(const double *U, float x[5]) {
 
 // Z0 = rho h W^2 = D W + Gamma/(Gamma-1) p W^2 from the network outputs
 AFD_D double mlp_guess_Z(double D, double gfac, const float y[2]) {
-    double p = D * exp10((double)(4.0f * y[0]));
-    double W = exp10((double)y[1]);
+    const float l2_10 = 3.32192809489f;  // log2(10)
+    double p = D * (double)__builtin_amdgcn_exp2f(4.0f * y[0] * l2_10);
+    double W = (double)__builtin_amdgcn_exp2f(y[1] * l2_10);
     if (!(W >= 1.0)) W = 1.0;
     return D * W + gfac * p * W * W;
 }

@@ -103,25 +103,49 @@ def open_domain(s, pr0, mlp=None):
     return d, rng
 
 
-def run_timed(s, pr0, steps, warmup, mlp=None):
-    """Warmup, then `steps` timed steps with per-kernel HIP events on the library's stream,
-    then one untimed step with the solver counters on."""
+def run_timed(s, pr0, steps, warmup, mlp=None, repeat=True):
+    """Time `steps` steps after `warmup` untimed ones, with per-kernel HIP events on the library's
+    stream, then one untimed step with the solver counters on.
+
+    repeat=True: every step starts from the seeded initial state (re-uploaded and filled
+    outside the timed interval), so each timed step is one pass of the hot path over the same
+    synthetic input.  repeat=False: consecutive steps of one run."""
     d, rng = open_domain(s, pr0, mlp)
     try:
         one_step = make_stepper(d, s, rng)
+
+        def reset():
+            if repeat:
+                d.set_prim(pr0)
+                d.apply_bcs(rng)
+
         for _ in range(warmup):
+            reset()
             one_step()
         d.set_instrumentation(stats=False, timing=True)
         d.reset_stats()
-        sync(d)
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            one_step()
-        sync(d)
-        elapsed = time.perf_counter() - t0
+        elapsed = 0.0
+        if repeat:
+            for _ in range(steps):
+                d.set_instrumentation(stats=False, timing=False)
+                reset()
+                d.set_instrumentation(stats=False, timing=True)
+                sync(d)
+                t0 = time.perf_counter()
+                one_step()
+                sync(d)
+                elapsed += time.perf_counter() - t0
+        else:
+            sync(d)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                one_step()
+            sync(d)
+            elapsed = time.perf_counter() - t0
         kern = d.stats()
         d.set_instrumentation(stats=True, timing=False)
         d.reset_stats()
+        reset()
         one_step()
         st = d.stats()
         d.set_instrumentation(stats=False, timing=False)
@@ -155,7 +179,29 @@ def accuracy(sp, sm, pr0, checkpoints, mlp=None):
     return out
 
 
-def kernel_roofline(s, kern, mixed):
+def measured_traffic(workload):
+    """HBM bytes per launch by kernel, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (profiles/<workload>_traffic.txt, scaled by a plain-copy calibration), or {}."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_traffic.txt")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f).get("traffic_bytes_per_launch", {})
+
+
+def kernel_name(s, which, mixed, learned):
+    """The rocprofv3 name of a kernel of this workload (face_x/y/z or stage_update_c2p)."""
+    T = "float" if mixed else "double"
+    nb = 9 if s["PHYSICS"] == "rmhd" else 5
+    if which == "stage_update_c2p":
+        return f"k_update<{nb}, {T}, {'true' if learned else 'false'}, true>"
+    from astrofd_amd.domain import recon_id, riemann_id
+    d = {"face_x": 0, "face_y": 1, "face_z": 2}[which]
+    kind = "k_face_rmhd" if nb == 9 else "k_face_srhd"
+    return f"{kind}<{recon_id(s['RECON'])}, {riemann_id(s['PHYSICS'], s['RIEMANN'])}, {d}, {T}>"
+
+
+def kernel_roofline(s, kern, mixed, learned=False, workload=None):
     """Dominant kernel and its achieved algorithmic HBM bandwidth (bytes per launch / avg duration).
 
     Algorithmic bytes per launch (compulsory traffic of one launch over its units):
@@ -187,9 +233,11 @@ def kernel_roofline(s, kern, mixed):
         cands.append((kern["ms_update"] / kern["launches_update"], "stage_update_c2p", upd_per_cell * full))
     ms, name, nbytes = max(cands)
     achieved = nbytes / (ms * 1e-3) / 1e9
+    kname = kernel_name(s, name, mixed, learned)
+    traffic = measured_traffic(workload).get(kname) if workload else None
     return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "avg_ms": round(ms, 4),
-            "bytes_per_launch": int(nbytes)}
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": round(traffic) if traffic else None,
+            "avg_ms": round(ms, 4), "bytes_per_launch": int(nbytes), "rocprof_name": kname}
 
 
 def cpu_baseline(s, n=64, steps=2, n1=32):
@@ -274,7 +322,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the FP64 plain run")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the lockstep accuracy pass")
-    ap.add_argument("--accuracy-steps", default="1,2,5,10,20")
+    ap.add_argument("--accuracy-steps", default="1,2,5,10")
+    ap.add_argument("--evolve", action="store_true",
+                    help="time consecutive steps of one run instead of steps from the seeded state")
+    ap.add_argument("--evolve-steps", type=int, default=10,
+                    help="consecutive steps timed beside the headline (0 = skip)")
     ap.add_argument("--copy-probe", type=float, default=0.0, help="GiB of a plain device copy (profiling)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -303,16 +355,16 @@ def main():
     plain = None
     if not a.no_plain:
         log("[bench] plain FP64 run")
-        plain = run_timed(sp, pr0, a.steps, a.warmup)
+        plain = run_timed(sp, pr0, a.steps, a.warmup, repeat=not a.evolve)
         log(f"[bench]   {plain['elapsed'] * 1e3 / a.steps:.2f} ms/step, {zu / plain['elapsed']:.4e} ZU/s")
     log(f"[bench] mixed run (c2p guess: {sm['C2P_GUESS']})")
-    mixed = run_timed(sm, pr0, a.steps, a.warmup, mlp=mlp)
+    mixed = run_timed(sm, pr0, a.steps, a.warmup, mlp=mlp, repeat=not a.evolve)
     log(f"[bench]   {mixed['elapsed'] * 1e3 / a.steps:.2f} ms/step, {zu / mixed['elapsed']:.4e} ZU/s")
 
     mixed_plain = None
     if mlp is not None:
         log("[bench] mixed run with the plain con2prim (A/B of the learned guess)")
-        mixed_plain = run_timed(dict(sm, C2P_GUESS="plain"), pr0, a.steps, a.warmup)
+        mixed_plain = run_timed(dict(sm, C2P_GUESS="plain"), pr0, a.steps, a.warmup, repeat=not a.evolve)
         log(f"[bench]   {mixed_plain['elapsed'] * 1e3 / a.steps:.2f} ms/step")
 
     value = zu / mixed["elapsed"]
@@ -324,11 +376,12 @@ def main():
         "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic (seeded IC: 1% density noise, Bx=0.1, sigma=0.1 toroidal nozzle, curl-A dB rms 1e-3)",
         "config": {"workload": f"{a.workload}: RMHD {s['NX']}^3 WENO5 + HLLD(hll_d_flux_dir) + SSPRK3, Gamma=4/3, "
-                               f"CFL {s['CFL']}, one step = fills + SSPRK3 + fills",
+                               f"CFL {s['CFL']}, one step = fills + SSPRK3 + fills, "
+                               + ("consecutive steps" if a.evolve else "every timed step from the seeded state"),
                    "grid": [s["NX"], s["NY"], s["NZ"]], "ng": s["NG"], "precision": "mixed",
                    "c2p_guess": sm["C2P_GUESS"]},
     }
-    res["roofline"] = kernel_roofline(s, mixed["kernels"], True)
+    res["roofline"] = kernel_roofline(s, mixed["kernels"], True, mlp is not None, a.workload)
     res["kernel_ms_per_step"] = kernel_ms(mixed["kernels"], a.steps)
     step_bytes = (13 if s["RK_ORDER"] == 3 else 8) * nb * 8
     res["step_roofline"] = {"bytes_per_zone_update": step_bytes, "achieved": round(step_bytes * value / 1e9, 2),
@@ -343,13 +396,25 @@ def main():
         res["fp64_plain"] = {"value": round(zu / plain["elapsed"], 1),
                              "ms_per_step": round(plain["elapsed"] * 1e3 / a.steps, 3),
                              "kernel_ms_per_step": kernel_ms(plain["kernels"], a.steps),
-                             "roofline": kernel_roofline(s, plain["kernels"], False),
+                             "roofline": kernel_roofline(s, plain["kernels"], False, False, a.workload),
                              "c2p": c2p_report(plain["stats"])}
         res["speedup_vs_fp64_plain"] = round(plain["elapsed"] / mixed["elapsed"], 3)
         if not a.no_accuracy:
-            cps = sorted({int(x) for x in a.accuracy_steps.split(",") if x.strip()})
+            cps = sorted({int(x) for x in a.accuracy_steps.split(",") if x.strip()} | {1})
             log(f"[bench] accuracy pass (steps {cps})")
-            res["errors_vs_fp64_plain"] = accuracy(sp, sm, pr0, set(cps), mlp)
+            ev = accuracy(sp, sm, pr0, set(cps), mlp)
+            # the headline: one step from the seeded state, as timed; later steps of the same run
+            # show how the reference scheme's own divergence (non-finite counts) takes over
+            res["errors_vs_fp64_plain"] = ev[0]
+            res["errors_evolution"] = ev
+        if a.evolve_steps > 0 and not a.evolve:
+            zu_e = s["NX"] * s["NY"] * s["NZ"] * a.evolve_steps
+            log(f"[bench] {a.evolve_steps} consecutive steps (plain, mixed)")
+            ep = run_timed(sp, pr0, a.evolve_steps, 0, repeat=False)
+            em = run_timed(sm, pr0, a.evolve_steps, 0, mlp=mlp, repeat=False)
+            res["consecutive_steps"] = {"steps": a.evolve_steps, "fp64_plain": round(zu_e / ep["elapsed"], 1),
+                                        "mixed": round(zu_e / em["elapsed"], 1),
+                                        "mixed_c2p": c2p_report(em["stats"])}
     if not a.no_cpu_baseline:
         log("[bench] CPU oracle sample")
         res["cpu_baseline"] = cpu_baseline(s)

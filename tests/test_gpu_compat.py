@@ -54,7 +54,8 @@ def test_srhd_core_calls_vs_oracle():
     ref = O.srhd_p2c(P, np.array([[1.0, 0.5, 0.1, 0.0, 0.1]]))[0]
     assert np.allclose([D, Sx, Sy, Sz, tau], ref, rtol=1e-14, atol=0)
     q = srhd_core.cons_to_prim(np.array([D, D]), Sx, Sy, Sz, tau)
-    assert q[0].shape == (2,) and np.allclose(q[1], 0.5, rtol=1e-9)
+    qo = O.srhd_c2p(P, np.array([[D, Sx, Sy, Sz, tau]]))[0]  # Q4: not the input primitives
+    assert q[0].shape == (2,) and np.allclose([v[0] for v in q], qo, rtol=1e-12, atol=0)
 
 
 def test_rmhd_core_calls_vs_oracle():
@@ -70,9 +71,12 @@ def test_rmhd_core_calls_vs_oracle():
     ref = O.step(P, pr, dx, dy, dz, 0.005)
     log(f"  step rel err {rel_err(out, ref):.2e}")
     assert rel_err(out, ref) <= 1e-8
-    U = rmhd_core.prim_to_cons_rmhd(1.0, 0.3, 0.0, 0.1, 0.05, 0.1, 0.2, 0.0, 0.0)
+    q0 = np.array([[1.0, 0.3, 0.0, 0.1, 0.05, 0.1, 0.2, 0.0, 0.0]])
+    U = rmhd_core.prim_to_cons_rmhd(*q0[0])
+    assert np.allclose(U, O.rmhd_p2c(P, q0)[0], rtol=1e-13, atol=1e-15)
     q = rmhd_core.cons_to_prim_rmhd(*U)
-    assert len(q) == 9 and abs(q[0] - 1.0) < 1e-8  # v.B != 0: the reference's p2c/c2p differ slightly
+    # v.B != 0: the reference's p2c and con2prim disagree, so q is not q0; compare with the oracle
+    assert len(q) == 9 and np.allclose(q, O.rmhd_c2p(P, np.array([U]))[0], rtol=1e-8, atol=1e-12)
 
 
 def test_boundary_and_nozzle_in_place_vs_oracle():
