@@ -441,7 +441,10 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(UpdArgs<F> a) {
 // wave-wide); `live` marks lanes with a cell.  Returns whether q holds the cell's primitives.
 template <int NB, bool LEARNED, typename F>
 AFD_D bool cell_c2p(const UpdArgs<F> &a, const double *U, bool live, long c, double *q, C2PStats &st) {
-    constexpr bool FAST = sizeof(F) == 4;  // the mixed path's con2prim uses the fast FP64 rcp/rsq
+    // The mixed RMHD con2prim uses the fast FP64 rcp/rsq: its hydro stage only seeds a Newton solve
+    // judged by its residual.  The SRHD con2prim's output is the fixed point's own branch (converged
+    // or the energy fallback), which ulp-level changes can flip, so it stays IEEE.
+    constexpr bool FAST = sizeof(F) == 4 && NB == 9;
     if constexpr (NB == 9) {
         RmhdC2PParams cp = {a.gamma, a.gfac, a.p_max, a.v_max};
         bool done = false;
