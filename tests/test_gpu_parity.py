@@ -184,8 +184,9 @@ def test_driver_golden(name):
     assert e_dt <= 1e-10
     if s["PHYSICS"] == "rmhd":
         # the RMHD FD-Newton stops at |f(Z)| < 1e-10 max(1, tau): FMA-perturbed iterates land
-        # elsewhere inside that tolerance, ~1e-6 relative in p of the coldest cells
-        assert err <= 1e-5 and l1 <= 1e-8
+        # elsewhere inside that tolerance, ~1e-6 relative in p of the coldest cells (where p << rho
+        # amplifies the relative error of Z); averaged over all cells and variables ~1e-8..1e-7
+        assert err <= 1e-5 and l1 <= 1e-7
     else:
         assert err <= 1e-8
 
