@@ -184,8 +184,9 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
     if (p.riemann_id < 0 || p.riemann_id > (nb == 9 ? 2 : 1)) return fail(AFD_ERR_INVALID, "riemann_id");
     if (p.rk_order != 2 && p.rk_order != 3) return fail(AFD_ERR_INVALID, "rk_order must be 2 or 3");
     if (p.precision != AFD_PREC_FP64 && p.precision != AFD_PREC_MIXED) return fail(AFD_ERR_INVALID, "precision");
-    if (p.c2p_guess != AFD_C2P_PLAIN && p.c2p_guess != AFD_C2P_LEARNED) return fail(AFD_ERR_INVALID, "c2p_guess");
-    if (p.c2p_guess == AFD_C2P_LEARNED && nb != 9) return fail(AFD_ERR_INVALID, "the learned guess is RMHD only");
+    if (p.c2p_guess != AFD_C2P_PLAIN && p.c2p_guess != AFD_C2P_LEARNED && p.c2p_guess != AFD_C2P_PREVIOUS)
+        return fail(AFD_ERR_INVALID, "c2p_guess");
+    if (p.c2p_guess != AFD_C2P_PLAIN && nb != 9) return fail(AFD_ERR_INVALID, "the con2prim guesses are RMHD only");
     if (p.two_temperature && (p.thermo_offset < nb || p.thermo_offset + 1 >= p.nv))
         return fail(AFD_ERR_INVALID, "thermo_offset outside the passive block");
     if (!(p.dx > 0 && p.dy > 0 && p.dz > 0)) return fail(AFD_ERR_INVALID, "dx, dy, dz must be > 0");
@@ -251,7 +252,7 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
     TRY(dalloc(c, (void **)&c->cnt_out, 8));
     TRY(dalloc(c, (void **)&c->stats_dev, sizeof(unsigned long long) * STATS_N * STATS_SLOTS));
     if (nb == 9) TRY(mlp_alloc(c->mlp, [c](void **ptr, size_t bytes) { return dalloc(c, ptr, bytes); }));
-    if (nb == 9 && p.c2p_guess == AFD_C2P_LEARNED) {
+    if (nb == 9 && p.c2p_guess != AFD_C2P_PLAIN) {
         long blocks = (long)grid1d(c->g.ncell).x;
         c->c2p_qcap = (blocks + QUEUE_SLOTS - 1) / QUEUE_SLOTS * BLOCK;
         TRY(dalloc(c, (void **)&c->c2p_queue, sizeof(int) * (size_t)c->c2p_qcap * QUEUE_SLOTS));
@@ -584,8 +585,9 @@ static UpdLaunch upd_launch(afd_ctx *c, const double *Pin, const double *P0, dou
     a.p_max = c->p.p_max;
     a.v_max = c->p.v_max;
     a.mlp = c->p.c2p_guess == AFD_C2P_LEARNED ? c->mlp.w : nullptr;
-    a.queue = a.mlp ? c->c2p_queue : nullptr;
-    a.qcount = a.mlp ? c->c2p_qcount : nullptr;
+    a.guess = c->p.c2p_guess;
+    a.queue = c->p.c2p_guess != AFD_C2P_PLAIN ? c->c2p_queue : nullptr;
+    a.qcount = c->p.c2p_guess != AFD_C2P_PLAIN ? c->c2p_qcount : nullptr;
     a.qcap = c->c2p_qcap;
     return l;
 }

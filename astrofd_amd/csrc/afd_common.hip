@@ -275,8 +275,8 @@ hipError_t launch_face_rmhd(const FaceLaunch &l, hipStream_t s) {
 #define AFD_SPLIT_C2P 0
 #endif
 template <int NB, typename F, bool LEARNED> static void upd_stage(const UpdArgs<F> &a, dim3 grid, hipStream_t s) {
-    if constexpr (LEARNED) {
-        if (a.queue) {  // rejected learned starts are queued and solved densely afterwards
+    if constexpr (NB == 9) {
+        if (a.queue) {  // rejected learned / previous-state starts are queued and solved densely afterwards
             (void)hipMemsetAsync(a.qcount, 0, sizeof(unsigned) * QUEUE_SLOTS, s);
             if constexpr (AFD_SPLIT_C2P) {
                 hipLaunchKernelGGL((k_update<NB, F, LEARNED, false>), grid, dim3(BLOCK), 0, s, a);

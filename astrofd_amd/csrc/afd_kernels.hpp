@@ -328,6 +328,7 @@ template <typename F> struct UpdArgs {
     double dt, dx, dy, dz, glm_cp2, resistivity;
     double gamma, gfac, p_max, v_max;
     const void *mlp;              // learned-guess weights (null: plain con2prim)
+    int guess;                    // 0 plain, 1 learned, 2 previous-state Newton start
     int *queue;                   // learned guess: cells the judge rejected, QUEUE_SLOTS regions
     unsigned *qcount;             //   of qcap entries each, filled per workgroup slot (or null)
     long qcap;
@@ -435,8 +436,16 @@ __global__ __launch_bounds__(BLOCK) void k_rhs(UpdArgs<F> a) {
     for (int t = 0; t < a.np; ++t) a.Pout[(NB + t) * g.ncell + c] = a.rhs_pass ? a.rhs_pass[t * g.ncell + c] : 0.0;
 }
 
+// Z = rho h W^2 of a primitive state (the previous-state Newton start)
+AFD_D double prim_Z(double gfac, double rho, double vx, double vy, double vz, double p) {
+    double v2 = vx * vx + vy * vy + vz * vz;
+    if (!(v2 < 1.0)) v2 = 1.0 - 1e-14;
+    return (rho + gfac * p) / (1.0 - v2);
+}
+
 // con2prim of one cell's conserved state U (FP64).  LEARNED: the RMHD Newton solve starts from
-// the network's guess, judged and backed by the plain solve: inline, or (queue set) by queueing
+// the network's guess; guess == 2: from Z of the stage input's primitives (P0 at stage 0, else
+// Pin).  Either start is judged and backed by the plain solve: inline, or (queue set) by queueing
 // the rejected cell for k_c2p_queue.  Reached by every lane of the wave (the network runs
 // wave-wide); `live` marks lanes with a cell.  Returns whether q holds the cell's primitives.
 template <int NB, bool LEARNED, typename F>
@@ -453,10 +462,17 @@ AFD_D bool cell_c2p(const UpdArgs<F> &a, const double *U, bool live, long c, dou
             bool fok = mlp_features(U, x) && live;
             mlp_infer_wave(mlp_view(a.mlp), x, y);
             if (fok) done = rmhd_c2p_from_guess<FAST>(cp, U, mlp_guess_Z(U[0], a.gfac, y), q, st);
-            if (a.queue) {
-                wave_enqueue(live && !done, c, a.queue, a.qcount, a.qcap);
-                return done;
+        } else {
+            if (a.guess == 2 && live) {
+                const double *P = (a.stage == 0 ? a.P0 : a.Pin) + c;
+                const long n = a.g.ncell;
+                done = rmhd_c2p_from_guess<FAST>(cp, U, prim_Z(a.gfac, P[0], P[n], P[2 * n], P[3 * n], P[4 * n]), q,
+                                                 st);
             }
+        }
+        if (a.queue) {
+            wave_enqueue(live && !done, c, a.queue, a.qcount, a.qcap);
+            return done;
         }
         if (live && !done) rmhd_c2p<FAST>(cp, U, q, st);
     } else {

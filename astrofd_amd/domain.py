@@ -72,7 +72,8 @@ def params_from_settings(s, device=0, **override):
     p.n_resist = n_resist_for(s, rc) if rm else 0
     p.resistivity = float(s.get("RESISTIVITY", 0.0))
     p.precision = L.AFD_PREC_MIXED if str(s.get("PRECISION", "fp64")).lower() == "mixed" else L.AFD_PREC_FP64
-    p.c2p_guess = L.AFD_C2P_LEARNED if str(s.get("C2P_GUESS", "plain")).lower() == "learned" else L.AFD_C2P_PLAIN
+    p.c2p_guess = {"learned": L.AFD_C2P_LEARNED, "previous": L.AFD_C2P_PREVIOUS}.get(
+        str(s.get("C2P_GUESS", "plain")).lower(), L.AFD_C2P_PLAIN)
     p.bc_y = L.AFD_BC_OUTFLOW if str(s.get("BC_Y", "periodic")).lower() == "outflow" else L.AFD_BC_PERIODIC
     p.bc_z = L.AFD_BC_OUTFLOW if str(s.get("BC_Z", "periodic")).lower() == "outflow" else L.AFD_BC_PERIODIC
     p.two_temperature = int(bool(s.get("TWO_TEMPERATURE", False)))

@@ -10,7 +10,8 @@ whole file when its json5 import fails (:4-11), this loader parses it.
 Additive keys (all default-off, so reference configs keep reference meaning):
   MAX_STEPS      stop after this many steps (besides T_END); 0 = unlimited
   PRECISION      "fp64" | "mixed" (FP32 reconstruction + Riemann fluxes)
-  C2P_GUESS      "plain" | "learned" (RMHD: MLP initial guess for the Newton solve)
+  C2P_GUESS      "plain" | "learned" (RMHD: MLP initial guess for the Newton solve) |
+                 "previous" (RMHD: Newton start from the stage input's primitives)
   C2P_MODEL      path of the learned-guess weights (.npz) for C2P_GUESS="learned"
   BC_Y, BC_Z     "periodic" | "outflow"
   IC_RHO_NOISE   amplitude of uniform density noise on interior cells (0 = none)

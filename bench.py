@@ -367,6 +367,12 @@ def main():
         mixed_plain = run_timed(dict(sm, C2P_GUESS="plain"), pr0, a.steps, a.warmup, repeat=not a.evolve)
         log(f"[bench]   {mixed_plain['elapsed'] * 1e3 / a.steps:.2f} ms/step")
 
+    mixed_prev = None
+    if s["PHYSICS"] == "rmhd":
+        log("[bench] mixed run with the previous-state Newton start")
+        mixed_prev = run_timed(dict(sm, C2P_GUESS="previous"), pr0, a.steps, a.warmup, repeat=not a.evolve)
+        log(f"[bench]   {mixed_prev['elapsed'] * 1e3 / a.steps:.2f} ms/step")
+
     value = zu / mixed["elapsed"]
     nb = 9 if s["PHYSICS"] == "rmhd" else 5
     res = {
@@ -392,6 +398,11 @@ def main():
                                   "ms_per_step": round(mixed_plain["elapsed"] * 1e3 / a.steps, 3),
                                   "kernel_ms_per_step": kernel_ms(mixed_plain["kernels"], a.steps),
                                   "c2p": c2p_report(mixed_plain["stats"])}
+    if mixed_prev is not None:
+        res["mixed_previous_guess"] = {"value": round(zu / mixed_prev["elapsed"], 1),
+                                       "ms_per_step": round(mixed_prev["elapsed"] * 1e3 / a.steps, 3),
+                                       "kernel_ms_per_step": kernel_ms(mixed_prev["kernels"], a.steps),
+                                       "c2p": c2p_report(mixed_prev["stats"])}
     if plain is not None:
         res["fp64_plain"] = {"value": round(zu / plain["elapsed"], 1),
                              "ms_per_step": round(plain["elapsed"] * 1e3 / a.steps, 3),

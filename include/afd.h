@@ -41,6 +41,7 @@ extern "C" {
 #define AFD_PREC_MIXED 1 /* reconstruction + Riemann in FP32; Lorentz factor, con2prim, SSPRK in FP64 */
 #define AFD_C2P_PLAIN 0
 #define AFD_C2P_LEARNED 1 /* MLP initial guess for the RMHD Newton solve; the plain solve judges and falls back */
+#define AFD_C2P_PREVIOUS 2 /* Newton start from the stage input's primitives (Z = rho h W^2); same judge and fallback */
 #define AFD_BC_PERIODIC 0
 #define AFD_BC_OUTFLOW 1
 

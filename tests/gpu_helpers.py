@@ -28,7 +28,7 @@ def make_domain(physics, pr_shape, ng, dxyz=(0.1, 0.1, 0.1), gamma=None, p_max=2
     p.n_resist = n_resist
     p.resistivity = resistivity
     p.precision = L.AFD_PREC_MIXED if precision == "mixed" else L.AFD_PREC_FP64
-    p.c2p_guess = L.AFD_C2P_LEARNED if c2p_guess == "learned" else L.AFD_C2P_PLAIN
+    p.c2p_guess = {"learned": L.AFD_C2P_LEARNED, "previous": L.AFD_C2P_PREVIOUS}.get(c2p_guess, L.AFD_C2P_PLAIN)
     p.bc_y = L.AFD_BC_OUTFLOW if bc_y == "outflow" else L.AFD_BC_PERIODIC
     p.bc_z = L.AFD_BC_OUTFLOW if bc_z == "outflow" else L.AFD_BC_PERIODIC
     p.two_temperature = int(two_temperature)

@@ -197,3 +197,25 @@ def test_learned_step_matches_plain():
     r = np.abs(a - b).max() / np.abs(a).max()
     log(f"  max rel diff {r:.3e}")
     assert r <= 1e-9
+
+
+def test_previous_state_guess_matches_plain():
+    """The previous-state Newton start (C2P_GUESS = previous) agrees with the plain con2prim."""
+    log("test_previous_state_guess_matches_plain")
+    from tests.golden_states import smooth_state
+    ng = 3
+    pr = smooth_state((20 + 2 * ng, 18 + 2 * ng, 16 + 2 * ng), 78, "rmhd", vamp=0.5)
+    outs = []
+    for guess in ("plain", "previous"):
+        d = make_domain("rmhd", pr.shape, ng, gamma=4.0 / 3.0, recon="weno", riemann="hlld_full", rk_order=3,
+                        c2p_guess=guess)
+        d.set_instrumentation(stats=True)
+        d.set_prim(pr)
+        d.step(1e-3)
+        outs.append(d.get_prim())
+        log(f"  {guess}: {d.stats()}")
+        d.close()
+    core = (slice(None),) + (slice(ng, -ng),) * 3
+    r = np.abs(outs[0][core] - outs[1][core]).max() / np.abs(outs[0][core]).max()
+    log(f"  max rel diff {r:.3e}")
+    assert r <= 1e-9
