@@ -269,31 +269,18 @@ hipError_t launch_face_rmhd(const FaceLaunch &l, hipStream_t s) {
     return hipGetLastError();
 }
 
-// The stage update runs fused (update + con2prim in one kernel) or split (update, then
-// k_c2p_cells over Uc); AFD_SPLIT_C2P selects the split form at build time.
-#ifndef AFD_SPLIT_C2P
-#define AFD_SPLIT_C2P 0
-#endif
+// One stage update: k_update (flux divergence, SSPRK, con2prim).  With a Newton start other than
+// the plain one, rejected cells are queued and solved densely by k_c2p_queue afterwards.
 template <int NB, typename F, bool LEARNED> static void upd_stage(const UpdArgs<F> &a, dim3 grid, hipStream_t s) {
     if constexpr (NB == 9) {
-        if (a.queue) {  // rejected learned / previous-state starts are queued and solved densely afterwards
+        if (a.queue) {
             (void)hipMemsetAsync(a.qcount, 0, sizeof(unsigned) * QUEUE_SLOTS, s);
-            if constexpr (AFD_SPLIT_C2P) {
-                hipLaunchKernelGGL((k_update<NB, F, LEARNED, false>), grid, dim3(BLOCK), 0, s, a);
-                hipLaunchKernelGGL((k_c2p_cells<NB, F, LEARNED>), grid, dim3(BLOCK), 0, s, a);
-            } else {
-                hipLaunchKernelGGL((k_update<NB, F, LEARNED, true>), grid, dim3(BLOCK), 0, s, a);
-            }
+            hipLaunchKernelGGL((k_update<NB, F, LEARNED>), grid, dim3(BLOCK), 0, s, a);
             hipLaunchKernelGGL((k_c2p_queue<F>), dim3(16, QUEUE_SLOTS), dim3(BLOCK), 0, s, a);
             return;
         }
     }
-    if constexpr (AFD_SPLIT_C2P) {
-        hipLaunchKernelGGL((k_update<NB, F, LEARNED, false>), grid, dim3(BLOCK), 0, s, a);
-        hipLaunchKernelGGL((k_c2p_cells<NB, F, LEARNED>), grid, dim3(BLOCK), 0, s, a);
-    } else {
-        hipLaunchKernelGGL((k_update<NB, F, LEARNED, true>), grid, dim3(BLOCK), 0, s, a);
-    }
+    hipLaunchKernelGGL((k_update<NB, F, LEARNED>), grid, dim3(BLOCK), 0, s, a);
 }
 
 template <int NB, typename F> static void upd(const UpdLaunch &l, bool rhs_only, hipStream_t s) {

@@ -482,10 +482,9 @@ AFD_D bool cell_c2p(const UpdArgs<F> &a, const double *U, bool live, long c, dou
 }
 
 // Stage update of one cell: flux divergence + sources, SSPRK combination into Uc (FP64), the
-// passives on primitives, and (C2P) the con2prim of the new state into Pout.  Without C2P the
-// con2prim runs in k_c2p_cells on Uc.  LEARNED selects the network's Newton start; a separate
-// instantiation keeps its registers out of the plain kernel.
-template <int NB, typename F, bool LEARNED, bool C2P = true>
+// passives on primitives, and the con2prim of the new state into Pout.  LEARNED selects the
+// network's Newton start; a separate instantiation keeps its registers out of the plain kernel.
+template <int NB, typename F, bool LEARNED>
 __global__ __launch_bounds__(BLOCK) void k_update(UpdArgs<F> a) {
     const Grid &g = a.g;
     const long c = (long)blockIdx.x * BLOCK + threadIdx.x;
@@ -521,8 +520,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(UpdArgs<F> a) {
             a.Uc[v * g.ncell + c] = U[v];
         }
     }
-    bool have = false;
-    if constexpr (C2P) have = cell_c2p<NB, LEARNED>(a, U, live, c, q, st);
+    const bool have = cell_c2p<NB, LEARNED>(a, U, live, c, q, st);
     if (live) {
         if (have) {
 #pragma unroll
@@ -538,25 +536,6 @@ __global__ __launch_bounds__(BLOCK) void k_update(UpdArgs<F> a) {
             }
             a.Pout[o] = ssprk_combine(a.stage, a.nst, a.P0[o], a.Pin[o], r);
         }
-    }
-    if constexpr (C2P) {
-        if (a.stats) wave_add_stats(a.stats, st);
-    }
-}
-
-// con2prim of every cell of Uc into Pout[0, NB) (the split form of the stage update).
-template <int NB, typename F, bool LEARNED>
-__global__ __launch_bounds__(BLOCK) void k_c2p_cells(UpdArgs<F> a) {
-    const long c = (long)blockIdx.x * BLOCK + threadIdx.x;
-    const long n = a.g.ncell;
-    const bool live = c < n;
-    C2PStats st = {0, 0, 0};
-    double U[NB], q[NB];
-#pragma unroll
-    for (int v = 0; v < NB; ++v) U[v] = live ? a.Uc[v * n + c] : 0.0;
-    if (cell_c2p<NB, LEARNED>(a, U, live, c, q, st)) {
-#pragma unroll
-        for (int v = 0; v < NB; ++v) a.Pout[v * n + c] = q[v];
     }
     if (a.stats) wave_add_stats(a.stats, st);
 }
