@@ -99,36 +99,38 @@ struct MlpTile {
     bf16x8 h1b[4], h2b[4];
 };
 
+// The biases enter as the MFMA's initial accumulator: lane l's element i is row drow(i, l).
+AFD_D floatx16 bias16(const float *b, int lane) {
+    floatx16 r;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) r[i] = b[drow(i, lane)];
+    return r;
+}
+
 AFD_D void mlp_tile_forward(const MlpDevW &w, int lane, bf16x8 xb, MlpTile &t) {
     floatx16 h[2];
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt) {
-        t.z1[mt] = mfma_bf16(w.blk[(BLK_W1 + mt) * 64 + lane], xb, zero16());
+        t.z1[mt] = mfma_bf16(w.blk[(BLK_W1 + mt) * 64 + lane], xb, bias16(w.bias + 32 * mt, lane));
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            t.z1[mt][i] += w.bias[32 * mt + drow(i, lane)];
-            h[mt][i] = silu(t.z1[mt][i]);
-        }
+        for (int i = 0; i < 16; ++i) h[mt][i] = silu(t.z1[mt][i]);
     }
     d_to_b(h, lane, t.h1b);
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt) {
-        floatx16 acc = zero16();
+        floatx16 acc = bias16(w.bias + 64 + 32 * mt, lane);
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb) acc = mfma_bf16(w.blk[(BLK_W2 + 4 * mt + kb) * 64 + lane], t.h1b[kb], acc);
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            acc[i] += w.bias[64 + 32 * mt + drow(i, lane)];
-            h[mt][i] = silu(acc[i]);
-        }
+        for (int i = 0; i < 16; ++i) h[mt][i] = silu(acc[i]);
         t.z2[mt] = acc;
     }
     d_to_b(h, lane, t.h2b);
     floatx16 o = zero16();
+    o[0] = w.bias[128];
+    o[1] = w.bias[129];
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) o = mfma_bf16(w.blk[(BLK_W3 + kb) * 64 + lane], t.h2b[kb], o);
-    o[0] += w.bias[128];
-    o[1] += w.bias[129];
     t.o = o;
 }
 
