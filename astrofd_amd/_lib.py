@@ -68,6 +68,11 @@ class Stats(ctypes.Structure):
         return d
 
 
+class Diag(ctypes.Structure):
+    _fields_ = [("max_gamma", _dbl), ("inlet_flux", _dbl), ("divb_max", _dbl), ("divb_rms", _dbl),
+                ("divb_mean", _dbl)]
+
+
 _VP = ctypes.c_void_p
 # name -> argtypes (all return int32)
 SIGNATURES = {
@@ -88,6 +93,8 @@ SIGNATURES = {
     "afd_count_nonfinite": [_VP, ctypes.POINTER(_i64)],
     "afd_p2c_batch": [_VP, _i64, _D, _D],
     "afd_c2p_batch": [_VP, _i64, _D, _D, ctypes.POINTER(_i32)],
+    "afd_diagnostics": [_VP, ctypes.POINTER(Diag)],
+    "afd_centerline": [_VP, _D, _i64],
     "afd_set_instrumentation": [_VP, _i32, _i32],
     "afd_get_stats": [_VP, ctypes.POINTER(Stats)],
     "afd_reset_stats": [_VP],

@@ -51,6 +51,7 @@ struct afd_ctx {
     int red_blocks = 1024;
     double *red_partial = nullptr, *red_out = nullptr;
     unsigned long long *cnt_partial = nullptr, *cnt_out = nullptr;
+    double *diag_partial = nullptr, *diag_out = nullptr;
     // instrumentation
     bool stats_on = false, timing_on = false;
     unsigned long long *stats_dev = nullptr;
@@ -250,6 +251,8 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
     TRY(dalloc(c, (void **)&c->red_out, 8));
     TRY(dalloc(c, (void **)&c->cnt_partial, c->red_blocks * 8));
     TRY(dalloc(c, (void **)&c->cnt_out, 8));
+    TRY(dalloc(c, (void **)&c->diag_partial, sizeof(double) * DIAG_N * c->red_blocks));
+    TRY(dalloc(c, (void **)&c->diag_out, sizeof(double) * DIAG_N));
     TRY(dalloc(c, (void **)&c->stats_dev, sizeof(unsigned long long) * STATS_N * STATS_SLOTS));
     if (nb == 9) TRY(mlp_alloc(c->mlp, [c](void **ptr, size_t bytes) { return dalloc(c, ptr, bytes); }));
     if (nb == 9 && p.c2p_guess != AFD_C2P_PLAIN) {
@@ -716,6 +719,47 @@ extern "C" int afd_p2c_batch(afd_ctx *c, int64_t n, const double *prim, double *
 }
 extern "C" int afd_c2p_batch(afd_ctx *c, int64_t n, const double *cons, double *prim, int32_t *iters) {
     return batch(c, n, cons, prim, iters, true);
+}
+
+// ------------------------------------------------------------------ diagnostics
+extern "C" int afd_diagnostics(afd_ctx *c, afd_diag *out) {
+    if (!c || !out) return fail(AFD_ERR_INVALID, "afd_diagnostics: null argument");
+    HIPCHK(hipSetDevice(c->p.device));
+    DiagArgs a{c->P[c->cur], c->diag_partial, c->g, c->p.ng, c->nb == 9 ? 1 : 0, c->p.dx, c->p.dy, c->p.dz,
+               (5.0 / 3.0) / (5.0 / 3.0 - 1.0), c->p.p_max, c->p.v_max};
+    {
+        Timed t(c, 4);
+        hipLaunchKernelGGL(k_diag, dim3(c->red_blocks), dim3(BLOCK), 0, c->stream, a);
+        hipLaunchKernelGGL(k_diag_reduce, dim3(1), dim3(BLOCK), 0, c->stream, (const double *)c->diag_partial,
+                           c->red_blocks, c->diag_out);
+    }
+    HIPCHK(hipGetLastError());
+    double v[DIAG_N];
+    HIPCHK(hipMemcpyAsync(v, c->diag_out, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const double ncell = (double)c->p.nx * c->p.ny * c->p.nz;
+    out->max_gamma = v[0];
+    out->inlet_flux = v[4];
+    out->divb_max = c->nb == 9 ? v[1] : 0.0;
+    out->divb_mean = c->nb == 9 ? v[2] / ncell : 0.0;
+    out->divb_rms = c->nb == 9 ? sqrt(v[3] / ncell) : 0.0;
+    return AFD_OK;
+}
+
+extern "C" int afd_centerline(afd_ctx *c, double *out, int64_t n) {
+    if (!c || !out) return fail(AFD_ERR_INVALID, "afd_centerline: null argument");
+    if (n != 4 * (int64_t)c->p.nx) return fail(AFD_ERR_INVALID, "afd_centerline: n must be 4 * nx");
+    HIPCHK(hipSetDevice(c->p.device));
+    double *d = nullptr;
+    HIPCHK(hipMalloc(&d, sizeof(double) * n));
+    hipLaunchKernelGGL(k_centerline, grid1d(c->p.nx), dim3(BLOCK), 0, c->stream, (const double *)c->P[c->cur], c->g,
+                       c->p.ng, c->p.v_max, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(AFD_ERR_HIP, std::string("centerline: ") + hipGetErrorString(e));
+    return AFD_OK;
 }
 
 // ------------------------------------------------------------------ instrumentation

@@ -309,3 +309,93 @@ hipError_t launch_update(const UpdLaunch &l, bool rhs_only, hipStream_t s) {
 }
 
 }  // namespace afd
+
+namespace afd {
+
+__global__ __launch_bounds__(BLOCK) void k_diag(DiagArgs a) {
+    const Grid &g = a.g;
+    __shared__ double red[DIAG_N][BLOCK];
+    const long ni = g.nx - 2 * a.ng, nj = g.ny - 2 * a.ng, nk = g.nz - 2 * a.ng, tot = ni * nj * nk;
+    double v[DIAG_N] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const long n = g.ncell;
+    for (long t = (long)blockIdx.x * BLOCK + threadIdx.x; t < tot; t += (long)gridDim.x * BLOCK) {
+        const int k = a.ng + (int)(t % nk), j = a.ng + (int)((t / nk) % nj), i = a.ng + (int)(t / (nj * nk));
+        const long c = cidx(g, i, j, k);
+        double vx = a.P[n + c], vy = a.P[2 * n + c], vz = a.P[3 * n + c];
+        double v2 = vx * vx + vy * vy + vz * vz;
+        if (v2 >= 1.0) v2 = 1.0 - 1e-14;
+        double W = 1.0 / sqrt(1.0 - v2);
+        if (W > v[0]) v[0] = W;
+        if (a.rmhd) {
+            double dBx = (a.P[5 * n + c + g.sx] - a.P[5 * n + c - g.sx]) / (2.0 * a.dx);
+            double dBy = (a.P[6 * n + c + g.sy] - a.P[6 * n + c - g.sy]) / (2.0 * a.dy);
+            double dBz = (a.P[7 * n + c + 1] - a.P[7 * n + c - 1]) / (2.0 * a.dz);
+            double d = dBx + dBy + dBz;
+            if (fabs(d) > v[1]) v[1] = fabs(d);
+            v[2] += d;
+            v[3] += d * d;
+        }
+        if (i == a.ng) {
+            double q[9], U[9];
+            for (int m = 0; m < (a.rmhd ? 9 : 5); ++m) q[m] = a.P[m * n + c];
+            if (a.rmhd) rmhd_p2c_cell(a.gfac53, a.p_max, a.v_max, q, U);
+            else srhd_p2c_cell(a.gfac53, q, U);
+            v[4] += U[1] * a.dy * a.dz;
+        }
+    }
+    for (int m = 0; m < DIAG_N; ++m) red[m][threadIdx.x] = v[m];
+    __syncthreads();
+    for (int o = BLOCK / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            red[0][threadIdx.x] = pymax(red[0][threadIdx.x], red[0][threadIdx.x + o]);
+            red[1][threadIdx.x] = pymax(red[1][threadIdx.x], red[1][threadIdx.x + o]);
+            for (int m = 2; m < DIAG_N; ++m) red[m][threadIdx.x] += red[m][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < DIAG_N) a.partial[threadIdx.x * gridDim.x + blockIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_diag_reduce(const double *partial, int nblocks, double *out) {
+    __shared__ double red[DIAG_N][BLOCK];
+    double v[DIAG_N] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int t = threadIdx.x; t < nblocks; t += BLOCK) {
+        v[0] = pymax(v[0], partial[t]);
+        v[1] = pymax(v[1], partial[nblocks + t]);
+        for (int m = 2; m < DIAG_N; ++m) v[m] += partial[m * nblocks + t];
+    }
+    for (int m = 0; m < DIAG_N; ++m) red[m][threadIdx.x] = v[m];
+    __syncthreads();
+    for (int o = BLOCK / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            red[0][threadIdx.x] = pymax(red[0][threadIdx.x], red[0][threadIdx.x + o]);
+            red[1][threadIdx.x] = pymax(red[1][threadIdx.x], red[1][threadIdx.x + o]);
+            for (int m = 2; m < DIAG_N; ++m) red[m][threadIdx.x] += red[m][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < DIAG_N) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_centerline(const double *P, Grid g, int ng, double v_max, double *out) {
+    const int nxi = g.nx - 2 * ng;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= nxi) return;
+    const int j = ng + (g.ny - 2 * ng) / 2, k = ng + (g.nz - 2 * ng) / 2;
+    const long n = g.ncell, c = cidx(g, ng + t, j, k);
+    double vx = P[n + c], vy = P[2 * n + c], vz = P[3 * n + c];
+    double v2 = vx * vx + vy * vy + vz * vz;
+    const double vmax2 = v_max * v_max;
+    if (v2 >= vmax2) {
+        double fac = v_max / sqrt(v2 + 1e-32);
+        vx *= fac;
+        v2 = vmax2 - 1e-14;
+    }
+    double v2c = v2 < 0.0 ? 0.0 : (v2 > 1.0 - 1e-14 ? 1.0 - 1e-14 : v2);
+    out[t] = 1.0 / sqrt(1.0 - v2c);
+    out[nxi + t] = P[c];
+    out[2 * nxi + t] = P[4 * n + c];
+    out[3 * nxi + t] = vx;
+}
+
+}  // namespace afd

@@ -630,6 +630,23 @@ __global__ void k_max_speed(MaxSpeedArgs a);
 __global__ void k_reduce_max(const double *partial, int n, double floor_value,
                                                        double *out);
 
+// Diagnostics of utils/diagnostics.py over the interior [ng, ng + n) per axis: maximum Lorentz
+// factor (:15-27), inlet flux sum of Sx dy dz on the plane i = ng (:31-43; the reference's
+// prim_to_cons* are called there without gamma, so their definition-time 5/3 applies) and, for
+// RMHD, the centred divergence of B (:143-153): max |divB|, sum, sum of squares.
+constexpr int DIAG_N = 5;  // maxW, max|divB|, sum divB, sum divB^2, inlet flux
+struct DiagArgs {
+    const double *P;
+    double *partial;  // DIAG_N x gridDim.x
+    Grid g;
+    int ng, rmhd;
+    double dx, dy, dz, gfac53, p_max, v_max;
+};
+__global__ void k_diag(DiagArgs a);
+__global__ void k_diag_reduce(const double *partial, int nblocks, double *out);
+// the mid-plane centerline (utils/diagnostics.py:59-126): Gamma, rho, p, vx at j, k = mid, i interior
+__global__ void k_centerline(const double *P, Grid g, int ng, double v_max, double *out);
+
 // count of non-finite values over n doubles; one partial count per block
 __global__ void k_count_nonfinite(const double *x, long n, unsigned long long *partial);
 

@@ -242,6 +242,18 @@ class Domain:
                                        it.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))), "afd_c2p_batch")
         return (out, it) if with_iters else out
 
+    # ---- diagnostics (utils/diagnostics.py)
+    def diagnostics(self):
+        d = L.Diag()
+        L.check(self.lib.afd_diagnostics(self.h, ctypes.byref(d)), "afd_diagnostics")
+        return {k: getattr(d, k) for k, _ in d._fields_}
+
+    def centerline(self):
+        """(Gamma, rho, p, vx) along x at the mid (j, k), interior cells: 4 x nx."""
+        out = np.empty((4, self.params.nx))
+        L.check(self.lib.afd_centerline(self.h, L.dptr(out), out.size), "afd_centerline")
+        return out
+
     # ---- instrumentation
     def set_instrumentation(self, stats=True, timing=False):
         L.check(self.lib.afd_set_instrumentation(self.h, int(stats), int(timing)), "afd_set_instrumentation")

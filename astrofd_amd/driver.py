@@ -5,7 +5,9 @@
 Same settings keys and CLI (utils/settings.py:100-107, plus --max-steps), the same loop
 (solvers/srhd3d_mpi_muscl.py:276-330): CFL time step (SRHD: max characteristic speed; RMHD:
 amax = 1), ghost/nozzle fills, one SSPRK2/3 step, fills again, post-step sources, progress
-print, NaN check, npz outputs and checkpoints with the reference's keys and meta.  The whole
+print, NaN check, npz outputs and checkpoints with the reference's keys and meta, and the
+diagnostics CSVs of utils/diagnostics.py (max Gamma and inlet flux, centerline, div B), whose
+reductions run on the GPU.  The whole
 domain is one GPU-resident block, so the MPI slab exchange (:62-126) has no counterpart: the
 fills provide every ghost zone.  The state stays in HBM between steps; it is copied to the
 host only for outputs, checkpoints and the NaN check.
@@ -89,10 +91,34 @@ def output_fields(s, pr):
     return base
 
 
-def max_lorentz(pr, ng):
-    sl = (slice(ng, -ng),) * 3
-    v2 = pr[1][sl] ** 2 + pr[2][sl] ** 2 + pr[3][sl] ** 2
-    return float(np.max(1.0 / np.sqrt(np.maximum(1.0 - v2, 1e-300))))
+def _append_csv(path, header, row):
+    new = not os.path.exists(path)
+    with open(path, "a") as f:
+        if new:
+            f.write(header + "\n")
+        f.write(row + "\n")
+
+
+def write_diagnostics(d, s, run_dir, step, t, dt, amax, dx):
+    """The reference's diagnostics.csv, centerline.csv and (RMHD) divb.csv rows
+    (utils/diagnostics.py:46-54, :111-126, :177-181), from the device reductions."""
+    g = d.diagnostics()
+    flux = g["inlet_flux"]
+    _append_csv(os.path.join(run_dir, "diagnostics.csv"), "step,time,dt,amax,maxGamma,inletFlux_signed,inletFlux_abs",
+                f"{step},{t:.8e},{dt:.8e},{amax:.6e},{g['max_gamma']:.6e},{flux:.6e},{abs(flux):.6e}")
+    cl = d.centerline()
+    path = os.path.join(run_dir, "centerline.csv")
+    new = not os.path.exists(path)
+    with open(path, "a") as f:
+        if new:
+            f.write("step,time,i,x,Gamma,rho,p,vx\n")
+        for i in range(cl.shape[1]):
+            f.write(f"{step},{t:.8e},{i},{(i + 0.5) * dx:.8e},{cl[0, i]:.8e},{cl[1, i]:.8e},{cl[2, i]:.8e},"
+                    f"{cl[3, i]:.8e}\n")
+    if s["PHYSICS"] == "rmhd":
+        _append_csv(os.path.join(run_dir, "divb.csv"), "step,time,divB_max,divB_rms,divB_mean",
+                    f"{step},{t:.8e},{g['divb_max']:.6e},{g['divb_rms']:.6e},{g['divb_mean']:.6e}")
+    return g
 
 
 def load_mlp(s):
@@ -166,7 +192,9 @@ def run(s, log=print):
                 np.savez(fname, **output_fields(s, pr), meta=np.array([dx, dy, dz, t], dtype=np.float64),
                          comment=kind)
                 log(f"[io] wrote {fname}")
-                log(f"[diag] step={step} maxGamma={max_lorentz(pr, NG):.3f}")
+                g = write_diagnostics(d, s, run_dir, step, t, dt, amax, dx)
+                log(f"[diag] step={step} maxGamma={g['max_gamma']:.3f} "
+                    f"inletFlux_signed={g['inlet_flux']:.3e} inletFlux_abs={abs(g['inlet_flux']):.3e}")
             ce = int(s.get("CHECKPOINT_EVERY", 0) or 0)
             if ce > 0 and step % ce == 0:
                 ckpt = os.path.join(run_dir, f"checkpoint_rank0000_step{step:06d}.npz")

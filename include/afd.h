@@ -102,6 +102,13 @@ typedef struct afd_stats {
     uint64_t launches_face[3], launches_update;
 } afd_stats;
 
+/* Run diagnostics of utils/diagnostics.py on the current state (interior cells). */
+typedef struct afd_diag {
+    double max_gamma;         /* max Lorentz factor (:15-27) */
+    double inlet_flux;        /* sum of Sx dy dz over the plane i = NG (:31-43; p2c with Gamma 5/3 as there) */
+    double divb_max, divb_rms, divb_mean; /* centred div B, RMHD only (:128-183) */
+} afd_diag;
+
 typedef struct afd_ctx afd_ctx;
 
 /* Context: replaces configure(params) (core/srhd_core.py:18, core/rmhd_core.py:21)
@@ -151,6 +158,11 @@ int afd_count_nonfinite(afd_ctx *ctx, int64_t *count);
  * (core/srhd_core.py:61, core/rmhd_core.py:248).  iters (optional, n): iterations used. */
 int afd_p2c_batch(afd_ctx *ctx, int64_t n, const double *prim, double *cons);
 int afd_c2p_batch(afd_ctx *ctx, int64_t n, const double *cons, double *prim, int32_t *iters);
+
+/* utils/diagnostics.py:5-57 (max Gamma, inlet flux) and :128-183 (div B), as device reductions. */
+int afd_diagnostics(afd_ctx *ctx, afd_diag *out);
+/* utils/diagnostics.py:59-126: the mid-plane centerline, out = 4 x nx (Gamma, rho, p, vx). */
+int afd_centerline(afd_ctx *ctx, double *out, int64_t n);
 
 /* Counters and per-kernel HIP-event timings (no counterpart in the reference). */
 int afd_set_instrumentation(afd_ctx *ctx, int32_t stats, int32_t timing);

@@ -127,6 +127,36 @@ def test_driver_matches_fixture(tmp_path, monkeypatch):
     err = rel_err(out, f["out"])
     log(f"  state rel err {err:.2e}; {lines[-3:]}")
     assert err <= 1e-8
-    written = sorted(os.listdir(os.path.join(tmp_path, "results", os.listdir(tmp_path / "results")[0])))
-    assert written == ["jet3d_rank0000_step000010.npz", "jet3d_rank0000_step000020.npz"]
+    run_dir = os.path.join(tmp_path, "results", os.listdir(tmp_path / "results")[0])
+    written = sorted(os.listdir(run_dir))
+    assert written == ["centerline.csv", "diagnostics.csv", "jet3d_rank0000_step000010.npz",
+                       "jet3d_rank0000_step000020.npz"]
+    rows = open(os.path.join(run_dir, "diagnostics.csv")).read().splitlines()
+    assert rows[0] == "step,time,dt,amax,maxGamma,inletFlux_signed,inletFlux_abs" and len(rows) == 3
     assert any(l.startswith("[rank0] t=") for l in lines) and lines[-1] == "Done."
+
+
+@pytest.mark.parametrize("phys", ["hydro", "rmhd"])
+def test_diagnostics_vs_oracle(phys):
+    """Device reductions of utils/diagnostics.py against the numpy oracle on seeded states."""
+    log(f"test_diagnostics_vs_oracle {phys}")
+    from oracle import diagnostics as D
+    from tests.gpu_helpers import make_domain
+    ng = 3
+    shape = (20 + 2 * ng, 18 + 2 * ng, 16 + 2 * ng)
+    pr = smooth_state(shape, 44, phys, vamp=0.6) if phys == "rmhd" else srhd_state(shape, 44)
+    d = make_domain(phys, pr.shape, ng, dxyz=(0.05, 0.04, 0.03), gamma=4.0 / 3.0 if phys == "rmhd" else None)
+    d.set_prim(pr)
+    g = d.diagnostics()
+    cl = d.centerline()
+    d.close()
+    ref = D.diagnostics(pr, ng, 0.05, 0.04, 0.03, rmhd=phys == "rmhd", p_max=2.0, v_max=0.9999)
+    log(f"  gpu {g}\n  ref {ref}")
+    # reductions in tree order and FMA-contracted squares: agreement to rounding, not bitwise
+    assert abs(g["max_gamma"] - ref["max_gamma"]) <= 1e-14 * ref["max_gamma"]
+    assert abs(g["inlet_flux"] - ref["inlet_flux"]) <= 1e-12 * max(abs(ref["inlet_flux"]), 1e-300)
+    if phys == "rmhd":
+        assert abs(g["divb_max"] - ref["divb_max"]) <= 1e-13 * ref["divb_max"]
+        assert abs(g["divb_rms"] - ref["divb_rms"]) <= 1e-12 * ref["divb_rms"]
+        assert abs(g["divb_mean"] - ref["divb_mean"]) <= 1e-10 * ref["divb_rms"]
+    np.testing.assert_allclose(cl, D.centerline(pr, ng, 0.9999), rtol=1e-14, atol=0)

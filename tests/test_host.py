@@ -118,3 +118,29 @@ def test_fixture_settings_readable():
     f = np.load(os.path.join(G, "driver_rmhd_8.npz"))
     s = ast.literal_eval(str(f["settings"]))
     assert s["PHYSICS"] == "rmhd" and s["RIEMANN"] == "hlld_full"
+
+
+def test_diagnostics_oracle_closed_forms():
+    """The diagnostics oracle on fields with known answers: a uniform moving state (max Gamma and
+    inlet flux in closed form) and a linear B field (div B = sum of its slopes everywhere)."""
+    from oracle import diagnostics as D
+    ng, n = 3, (10, 8, 6)
+    pr = np.zeros((9,) + tuple(m + 2 * ng for m in n))
+    pr[0], pr[1], pr[4] = 2.0, 0.6, 0.1
+    x = np.arange(pr.shape[1])[:, None, None] * 0.1
+    y = np.arange(pr.shape[2])[None, :, None] * 0.1
+    z = np.arange(pr.shape[3])[None, None, :] * 0.1
+    pr[5] = 3.0 * x + 0 * y + 0 * z
+    pr[6] = -1.0 * y + 0 * x + 0 * z
+    pr[7] = 0.5 * z + 0 * x + 0 * y
+    d = D.diagnostics(pr, ng, 0.1, 0.1, 0.1, rmhd=True, p_max=2.0, v_max=0.9999)
+    W = 1.0 / np.sqrt(1.0 - 0.36)
+    assert abs(d["max_gamma"] - W) < 1e-14
+    assert abs(d["divb_mean"] - 2.5) < 1e-12 and abs(d["divb_max"] - 2.5) < 1e-12
+    assert abs(d["divb_rms"] - 2.5) < 1e-12
+    # SRHD inlet flux: Sx = rho h W^2 vx (Gamma 5/3), summed over the ny*nz inlet cells
+    h = 1.0 + 2.5 * 0.1 / 2.0
+    ds = D.diagnostics(pr[:5], ng, 0.1, 0.1, 0.1, rmhd=False)
+    assert abs(ds["inlet_flux"] - 2.0 * h * W * W * 0.6 * 0.01 * n[1] * n[2]) < 1e-12
+    c = D.centerline(pr, ng, 0.9999)
+    assert c.shape == (4, n[0]) and np.allclose(c[0], W) and np.allclose(c[3], 0.6)
