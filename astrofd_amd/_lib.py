@@ -49,7 +49,18 @@ class Nozzle(ctypes.Structure):
         ("tracer_nozzle", _dbl * 8), ("tracer_amb", _dbl * 8),
         ("two_temperature", _i32), ("thermo_offset", _i32), ("te_nozzle", _dbl), ("ti_nozzle", _dbl),
         ("b_init", _i32), ("b0", _dbl), ("b_amb", _dbl * 3),
+        ("chemistry", _i32), ("chem_offset", _i32), ("chem_nozzle", _dbl * 3),
     ]
+
+
+class Chem(ctypes.Structure):
+    _fields_ = [("offset", _i32), ("X", _dbl), ("Y", _dbl), ("tunit_k", _dbl), ("rate_scale", _dbl),
+                ("energy_scale", _dbl), ("tmin_k", _dbl), ("gamma", _dbl)]
+
+
+class Sources(ctypes.Structure):
+    _fields_ = [("cooling", _i32), ("cool", _dbl), ("heat", _dbl), ("p_cap", _dbl),
+                ("chemistry", _i32), ("chem", Chem)]
 
 
 class Stats(ctypes.Structure):
@@ -90,6 +101,9 @@ SIGNATURES = {
     "afd_step": [_VP, _dbl],
     "afd_compute_rhs": [_VP, _D, _i64],
     "afd_two_temperature": [_VP, _dbl],
+    "afd_cooling_heating": [_VP, _dbl, _dbl, _dbl, _dbl],
+    "afd_ion_chemistry": [_VP, _dbl, ctypes.POINTER(Chem)],
+    "afd_post_step_sources": [_VP, _dbl, ctypes.POINTER(Sources)],
     "afd_count_nonfinite": [_VP, ctypes.POINTER(_i64)],
     "afd_p2c_batch": [_VP, _i64, _D, _D],
     "afd_c2p_batch": [_VP, _i64, _D, _D, ctypes.POINTER(_i32)],

@@ -48,6 +48,24 @@ def domain(physics, full_shape, dxyz=(1.0, 1.0, 1.0), rk_order=2, ng=None, gamma
     return d
 
 
+def source_domain(full_shape, ng, two_temp=None):
+    """A context for the post-step sources on arrays of `full_shape`.  They read their constants
+    from the cfg of each call (core/source_terms.py, core/chemistry.py), so no configure() is
+    needed; `two_temp` = (THERMO_OFFSET, TEI_TAU) or None."""
+    nv, fx, fy, fz = (int(v) for v in full_shape)
+    key = ("sources", nv, fx, fy, fz, int(ng), two_temp)
+    d = _CACHE.get(key)
+    if d is None:
+        nx, ny, nz = max(fx - 2 * ng, 1), max(fy - 2 * ng, 1), max(fz - 2 * ng, 1)
+        s = dict(PHYSICS="hydro", NX=nx, NY=ny, NZ=nz, NG=int(ng), Lx=1.0, Ly=1.0, Lz=1.0)
+        over = dict(nv=nv, n_passive=nv - 5)
+        if two_temp is not None:
+            over.update(two_temperature=1, thermo_offset=int(two_temp[0]), tei_tau=float(two_temp[1]))
+        d = Domain(params_from_settings(s, **over))
+        _CACHE[key] = d
+    return d
+
+
 def pointwise(physics, gamma=None):
     """A small context for the pointwise conversions (its grid is unused)."""
     return domain(physics, (9 if physics == "rmhd" else 5, 8, 8, 8), ng=2, gamma=gamma)
@@ -72,4 +90,4 @@ def close_all():
     _CACHE.clear()
 
 
-__all__ = ["configure", "settings", "domain", "pointwise", "as_batch", "unbatch", "close_all", "L"]
+__all__ = ["configure", "settings", "domain", "source_domain", "pointwise", "as_batch", "unbatch", "close_all", "L"]

@@ -353,6 +353,11 @@ extern "C" int afd_set_nozzle(afd_ctx *c, const afd_nozzle *nz) {
     for (int t = 0; t < nz->n_tracers; ++t) mask |= 1u << (nz->tracer_offset + t);
     bool tt = nz->two_temperature && p.nv > nz->thermo_offset + 1;
     if (tt) mask |= (1u << nz->thermo_offset) | (1u << (nz->thermo_offset + 1));
+    if (nz->chemistry && (nz->chem_offset < c->nb || nz->chem_offset > 29))
+        return fail(AFD_ERR_INVALID, "chemistry offset outside [nb, 29]");
+    bool chem = nz->chemistry && p.nv > nz->chem_offset + 2;  // core/nozzle.py:77-82
+    if (chem) mask |= 7u << nz->chem_offset;
+    if (nv >= 15) mask |= 0x7fe0u;  // core/nozzle.py:60-61: pr[5:15] = 0
     if (c->nb == 9) mask |= 0x1e0u;
     int nact = 0;
     double beta = sqrt(1.0 - 1.0 / (nz->gamma_jet * nz->gamma_jet));
@@ -376,6 +381,8 @@ extern "C" int afd_set_nozzle(afd_ctx *c, const afd_nozzle *nz) {
                 tmpl[(size_t)nz->thermo_offset * plane + o] = nz->te_nozzle;
                 tmpl[(size_t)(nz->thermo_offset + 1) * plane + o] = nz->ti_nozzle;
             }
+            if (chem)
+                for (int m = 0; m < 3; ++m) tmpl[(size_t)(nz->chem_offset + m) * plane + o] = nz->chem_nozzle[m];
             if (c->nb == 9) {
                 double Bx = 0.0, By = 0.0, Bz = 0.0;
                 if (nz->b_init == 1) Bx = nz->b0 * s;
@@ -636,15 +643,106 @@ extern "C" int afd_compute_rhs(afd_ctx *c, double *host, int64_t n) {
     return AFD_OK;
 }
 
-extern "C" int afd_two_temperature(afd_ctx *c, double dt) {
-    if (!c) return fail(AFD_ERR_INVALID, "null context");
-    if (!c->p.two_temperature || c->p.tei_tau <= 0.0) return AFD_OK;  // core/source_terms.py:34-42
+// One pass of k_sources over the interior (the kernel applies the SRC_* bits in the driver's order).
+static int run_sources(afd_ctx *c, SourceArgs &a) {
+    if (!a.which) return AFD_OK;
+    const int ng = c->p.ng;
+    a.P = c->P[c->cur];
+    a.g = c->g;
+    a.ng = ng;
     HIPCHK(hipSetDevice(c->p.device));
-    TwoTempArgs a{c->P[c->cur], c->g, c->p.ng, c->p.thermo_offset, dt, c->p.tei_tau};
+    dim3 grid((unsigned)((long)c->p.nx * c->p.ny), (unsigned)((c->p.nz + BLOCK - 1) / BLOCK));
     Timed t(c, 4);
-    hipLaunchKernelGGL(k_two_temperature, grid1d(c->g.ncell), dim3(BLOCK), 0, c->stream, a);
+    hipLaunchKernelGGL(k_sources, grid, dim3(BLOCK), 0, c->stream, a);
     HIPCHK(hipGetLastError());
     return AFD_OK;
+}
+
+static bool two_temp_on(const afd_ctx *c) {  // core/source_terms.py:31-39
+    return c->p.two_temperature && c->p.nv >= c->p.thermo_offset + 2 && c->p.tei_tau > 0.0;
+}
+
+static int check_chem(const afd_ctx *c, const afd_chem *ch, bool *on) {
+    if (ch->offset < c->nb) return fail(AFD_ERR_INVALID, "chemistry offset overlaps the base variables");
+    *on = c->p.nv > ch->offset + 2;  // core/chemistry.py:55-57
+    return AFD_OK;
+}
+
+static void set_chem(SourceArgs &a, const afd_chem *ch) {
+    a.c_off = ch->offset;
+    a.X = ch->X;
+    a.Y = ch->Y;
+    a.tunit = ch->tunit_k;
+    a.rate_scale = ch->rate_scale;
+    a.energy_scale = ch->energy_scale;
+    a.tmin = ch->tmin_k;
+    a.gamma = ch->gamma;
+}
+
+extern "C" int afd_two_temperature(afd_ctx *c, double dt) {
+    if (!c) return fail(AFD_ERR_INVALID, "null context");
+    SourceArgs a{};
+    a.dt = dt;
+    if (two_temp_on(c)) {
+        a.which = SRC_TWO_TEMP;
+        a.t_off = c->p.thermo_offset;
+        a.tau = c->p.tei_tau;
+    }
+    return run_sources(c, a);
+}
+
+extern "C" int afd_cooling_heating(afd_ctx *c, double dt, double cool, double heat, double p_cap) {
+    if (!c) return fail(AFD_ERR_INVALID, "null context");
+    SourceArgs a{};
+    a.dt = dt;
+    if (cool != 0.0 || heat != 0.0) {  // core/source_terms.py:10-11
+        a.which = SRC_COOL;
+        a.cool = cool;
+        a.heat = heat;
+        a.p_cap = p_cap;
+    }
+    return run_sources(c, a);
+}
+
+extern "C" int afd_ion_chemistry(afd_ctx *c, double dt, const afd_chem *chem) {
+    if (!c || !chem) return fail(AFD_ERR_INVALID, "afd_ion_chemistry: null argument");
+    bool on = false;
+    int rc = check_chem(c, chem, &on);
+    if (rc) return rc;
+    SourceArgs a{};
+    a.dt = dt;
+    if (on) {
+        a.which = SRC_CHEM;
+        set_chem(a, chem);
+    }
+    return run_sources(c, a);
+}
+
+extern "C" int afd_post_step_sources(afd_ctx *c, double dt, const afd_sources *src) {
+    if (!c || !src) return fail(AFD_ERR_INVALID, "afd_post_step_sources: null argument");
+    SourceArgs a{};
+    a.dt = dt;
+    if (src->cooling && (src->cool != 0.0 || src->heat != 0.0)) {
+        a.which |= SRC_COOL;
+        a.cool = src->cool;
+        a.heat = src->heat;
+        a.p_cap = src->p_cap;
+    }
+    if (two_temp_on(c)) {
+        a.which |= SRC_TWO_TEMP;
+        a.t_off = c->p.thermo_offset;
+        a.tau = c->p.tei_tau;
+    }
+    if (src->chemistry) {
+        bool on = false;
+        int rc = check_chem(c, &src->chem, &on);
+        if (rc) return rc;
+        if (on) {
+            a.which |= SRC_CHEM;
+            set_chem(a, &src->chem);
+        }
+    }
+    return run_sources(c, a);
 }
 
 extern "C" int afd_max_char_speed(afd_ctx *c, double *amax) {

@@ -172,21 +172,77 @@ __global__ __launch_bounds__(BLOCK) void k_nozzle(NozzleArgs a) {
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_two_temperature(TwoTempArgs a) {
+// core/chemistry.py:17-27: collisional ionization fit, T in K
+__device__ static double rate_ci(double a, double e, double T) {
+#pragma clang fp contract(off)
+    return a * sqrt(T) * exp(-e / pymax(T, 1.0)) / (1.0 + sqrt(T / 1.0e5));
+}
+
+__device__ static double clamp01(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+__global__ __launch_bounds__(BLOCK) void k_sources(SourceArgs a) {
 #pragma clang fp contract(off)
     const Grid &g = a.g;
-    long c = (long)blockIdx.x * BLOCK + threadIdx.x;
-    if (c >= g.ncell) return;
-    int k = (int)(c % g.nz), j = (int)((c / g.nz) % g.ny), i = (int)(c / ((long)g.ny * g.nz));
-    if (i < a.ng || i >= g.nx - a.ng || j < a.ng || j >= g.ny - a.ng || k < a.ng || k >= g.nz - a.ng) return;
-    double rho = a.P[c], p = a.P[4 * g.ncell + c];
-    double T = p / pymax(rho, 1e-12);
-    double *Te = a.P + (long)a.off * g.ncell + c, *Ti = a.P + (long)(a.off + 1) * g.ncell + c;
-    double te = *Te, ti = *Ti;
-    te += a.dt * (T - te) / a.tau;
-    ti += a.dt * (T - ti) / a.tau;
-    *Te = te;
-    *Ti = ti;
+    const int nyi = g.ny - 2 * a.ng, nzi = g.nz - 2 * a.ng;
+    const int kk = blockIdx.y * BLOCK + threadIdx.x;
+    if (kk >= nzi) return;
+    const int i = a.ng + (int)blockIdx.x / nyi, j = a.ng + (int)blockIdx.x % nyi, k = a.ng + kk;
+    const long c = cidx(g, i, j, k), n = g.ncell;
+    double *P = a.P;
+    const double rho = P[c];
+    double p = P[4 * n + c];
+    if (a.which & SRC_COOL) {  // core/source_terms.py:19-27
+        double p_new = p + a.dt * (a.heat - a.cool * rho * p);
+        if (p_new < 1e-12) p_new = 1e-12;
+        if (p_new > a.p_cap) p_new = a.p_cap;
+        p = p_new;
+        P[4 * n + c] = p;
+    }
+    if (a.which & SRC_TWO_TEMP) {  // core/source_terms.py:45-54
+        double T = p / pymax(rho, 1e-12);
+        double *Te = P + (long)a.t_off * n + c, *Ti = Te + n;
+        double te = *Te, ti = *Ti;
+        te += a.dt * (T - te) / a.tau;
+        ti += a.dt * (T - ti) / a.tau;
+        *Te = te;
+        *Ti = ti;
+    }
+    if ((a.which & SRC_CHEM) && rho > 0.0 && p > 0.0) {  // core/chemistry.py:78-137
+        const double SM = 1e-20;
+        double T = (p / pymax(rho, SM)) * a.tunit;
+        if (T < a.tmin) T = a.tmin;
+        double nH = a.X * rho, nHe = a.Y * rho / 4.0;
+        double *x = P + (long)a.c_off * n + c;
+        double xHII = clamp01(x[0], 0.0, 1.0), xHeII = clamp01(x[n], 0.0, 1.0), xHeIII = clamp01(x[2 * n], 0.0, 1.0);
+        xHeII = pymin(xHeII, 1.0 - xHeIII);
+        double xHeI = pymax(0.0, 1.0 - xHeII - xHeIII);
+        double xHI = 1.0 - xHII;
+        double ne = pymax(nH * xHII + nHe * (xHeII + 2.0 * xHeIII), SM);
+        double C_H = a.rate_scale * rate_ci(5.85e-11, 157809.1, T);
+        double C_HeI = a.rate_scale * rate_ci(2.38e-11, 285335.4, T);
+        double C_HeII = a.rate_scale * rate_ci(5.68e-12, 631515.0, T);
+        // case-B recombination, core/chemistry.py:30-40: (T / 1e4)^y as exp(y log(T / 1e4)), one
+        // log for the three powers; |y log| <= ~20 over T in [1, 1e12] K, so the result is within
+        // ~20 ulp of the correctly rounded power
+        double lt4 = log(T / 1.0e4);
+        double A_H = a.rate_scale * (2.59e-13 * exp(-0.7 * lt4));
+        double A_HeII = a.rate_scale * (1.50e-10 * exp(-0.6353 * lt4));
+        double A_HeIII = a.rate_scale * (3.36e-10 * exp(-0.5 * lt4));
+        double dne = a.dt * ne;
+        double xHII_new = (xHII + dne * C_H * xHI) / (1.0 + dne * A_H);
+        double xHeIII_new = clamp01((xHeIII + dne * C_HeII * xHeII) / (1.0 + dne * A_HeIII), 0.0, 1.0);
+        double xHeII_new = (xHeII + dne * (C_HeI * xHeI + A_HeIII * xHeIII_new)) / (1.0 + dne * (A_HeII + C_HeII));
+        xHII_new = clamp01(xHII_new, 0.0, 1.0);
+        xHeII_new = clamp01(xHeII_new, 0.0, 1.0 - xHeIII_new);
+        double dE = a.energy_scale * (nH * 157809.1 * (xHII_new - xHII) + nHe * 285335.4 * (xHeII_new - xHeII) +
+                                      nHe * 631515.0 * (xHeIII_new - xHeIII));
+        double p_new = p + -(a.gamma - 1.0) * dE / pymax(a.tunit, SM);
+        if (p_new < SM) p_new = SM;
+        P[4 * n + c] = p_new;
+        x[0] = xHII_new;
+        x[n] = xHeII_new;
+        x[2 * n] = xHeIII_new;
+    }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_max_speed(MaxSpeedArgs a) {

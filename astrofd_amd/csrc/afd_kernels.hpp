@@ -608,14 +608,24 @@ struct NozzleArgs {
 
 __global__ void k_nozzle(NozzleArgs a);
 
-// core/source_terms.py:30-55 (interior cells)
-struct TwoTempArgs {
+// The driver's post-step sources (solvers/srhd3d_mpi_muscl.py:324-326), pointwise over the
+// interior, applied in that order to each cell in one pass: cooling/heating
+// (core/source_terms.py:5-28), two-temperature relaxation (:30-55), H/He ion chemistry
+// (core/chemistry.py:43-139).  One thread per cell of a k row; blockIdx.x = interior (i, j),
+// blockIdx.y = chunk of k.
+enum { SRC_COOL = 1, SRC_TWO_TEMP = 2, SRC_CHEM = 4 };
+struct SourceArgs {
     double *P;
     Grid g;
-    int ng, off;
-    double dt, tau;
+    int ng, which;             // SRC_* bits
+    double dt;
+    double cool, heat, p_cap;  // COOLING_LAMBDA, HEATING_RATE, P_MAX
+    int t_off;                 // Te, Ti
+    double tau;                // TEI_TAU
+    int c_off;                 // xHII, xHeII, xHeIII
+    double X, Y, tunit, rate_scale, energy_scale, tmin, gamma;
 };
-__global__ void k_two_temperature(TwoTempArgs a);
+__global__ void k_sources(SourceArgs a);
 
 // ------------------------------------------------------------------ reductions
 // core/srhd_core.py:1267-1281 over cells [1, n-1)^3; one partial maximum per block

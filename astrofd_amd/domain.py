@@ -43,9 +43,8 @@ def params_from_settings(s, device=0, **override):
     phys = str(s.get("PHYSICS", "hydro")).lower()
     if phys not in ("hydro", "rmhd"):
         raise NotImplementedError(f"PHYSICS={phys!r}: only the SRHD (hydro) and RMHD step are on this path")
-    for key in ("DISSIPATION_ENABLED", "CHEMISTRY_ENABLED", "COOLING_ENABLED"):
-        if s.get(key, False):
-            raise NotImplementedError(f"{key}: the post-step operator is not on this path")
+    if s.get("DISSIPATION_ENABLED", False):
+        raise NotImplementedError("DISSIPATION_ENABLED: the dissipative operator is not on this path")
     rm = phys == "rmhd"
     nb = 9 if rm else 5
     rc = recon_id(s.get("RECON", "muscl"))
@@ -122,7 +121,33 @@ def nozzle_from_settings(s):
     bamb = s.get("B_AMB") or [0.0, 0.0, 0.0]
     for m in range(3):
         n.b_amb[m] = float(bamb[m])
+    if s.get("CHEMISTRY_ENABLED", False):  # core/nozzle.py:77-82
+        n.chemistry, n.chem_offset = 1, int(s.get("CHEM_OFFSET", 0))
+        for m, key in enumerate(("CHEM_X_HII_NOZZLE", "CHEM_X_HEII_NOZZLE", "CHEM_X_HEIII_NOZZLE")):
+            v = s.get(key, 0.0)
+            n.chem_nozzle[m] = float(v if v is not None else 0.0)
     return n
+
+
+def chem_from_settings(s):
+    """The constants core/chemistry.py:59-68 reads from the settings."""
+    return L.Chem(offset=int(s.get("CHEM_OFFSET", 0)), X=float(s.get("CHEM_X", 0.76)), Y=float(s.get("CHEM_Y", 0.24)),
+                  tunit_k=float(s.get("CHEM_TUNIT_K", 1.0e4)), rate_scale=float(s.get("CHEM_RATE_SCALE", 1.0)),
+                  energy_scale=float(s.get("CHEM_ENERGY_SCALE", 1.0)), tmin_k=float(s.get("CHEM_TMIN_K", 1.0)),
+                  gamma=float(s.get("GAMMA", 5.0 / 3.0)))
+
+
+def sources_from_settings(s):
+    """The post-step sources the driver applies (solvers/srhd3d_mpi_muscl.py:324-326) and their
+    settings gates: core/source_terms.py:6-11, core/chemistry.py:48-53."""
+    src = L.Sources()
+    src.cooling = int(bool(s.get("COOLING_ENABLED", False)))
+    src.cool = float(s.get("COOLING_LAMBDA", 0.0))
+    src.heat = float(s.get("HEATING_RATE", 0.0))
+    src.p_cap = float(s.get("P_MAX", 1.0))
+    src.chemistry = int(bool(s.get("CHEMISTRY_ENABLED", False)) and int(s.get("N_CHEM", 0)) >= 3)
+    src.chem = chem_from_settings(s)
+    return src
 
 
 class Domain:
@@ -138,10 +163,13 @@ class Domain:
         L.check(self.lib.afd_create(ctypes.byref(params), ctypes.byref(h)), "afd_create")
         self.h = h
         self.nozzle_set = False
+        self.sources = L.Sources()  # post-step sources beside two-temperature: none unless from settings
 
     @classmethod
     def from_settings(cls, s, device=0, **override):
-        return cls(params_from_settings(s, device, **override))
+        d = cls(params_from_settings(s, device, **override))
+        d.sources = sources_from_settings(s)
+        return d
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -222,6 +250,21 @@ class Domain:
 
     def two_temperature(self, dt):
         L.check(self.lib.afd_two_temperature(self.h, float(dt)), "afd_two_temperature")
+
+    def cooling_heating(self, dt, cool, heat, p_cap=1.0):
+        L.check(self.lib.afd_cooling_heating(self.h, float(dt), float(cool), float(heat), float(p_cap)),
+                "afd_cooling_heating")
+
+    def ion_chemistry(self, dt, chem):
+        """`chem`: an L.Chem, or a settings dict (chem_from_settings)."""
+        if isinstance(chem, dict):
+            chem = chem_from_settings(chem)
+        L.check(self.lib.afd_ion_chemistry(self.h, float(dt), ctypes.byref(chem)), "afd_ion_chemistry")
+
+    def post_step_sources(self, dt, sources=None):
+        """Cooling/heating, two-temperature and chemistry in one pass (the driver's order)."""
+        src = self.sources if sources is None else sources
+        L.check(self.lib.afd_post_step_sources(self.h, float(dt), ctypes.byref(src)), "afd_post_step_sources")
 
     def count_nonfinite(self):
         n = ctypes.c_int64()

@@ -4,7 +4,8 @@
 
 Same settings keys and CLI (utils/settings.py:100-107, plus --max-steps), the same loop
 (solvers/srhd3d_mpi_muscl.py:276-330): CFL time step (SRHD: max characteristic speed; RMHD:
-amax = 1), ghost/nozzle fills, one SSPRK2/3 step, fills again, post-step sources, progress
+amax = 1), ghost/nozzle fills, one SSPRK2/3 step, fills again, post-step sources (cooling/heating,
+two-temperature, ion chemistry, fused into one pass over the interior), progress
 print, NaN check, npz outputs and checkpoints with the reference's keys and meta, and the
 diagnostics CSVs of utils/diagnostics.py (max Gamma and inlet flux, centerline, div B), whose
 reductions run on the GPU.  The whole
@@ -84,6 +85,10 @@ def output_fields(s, pr):
     if int(s.get("N_THERMO", 0)) > 0:
         f["Te"] = pr[int(s["THERMO_OFFSET"])]
         f["Ti"] = pr[int(s["THERMO_OFFSET"]) + 1]
+    if int(s.get("N_CHEM", 0)) > 0:
+        names = s.get("CHEM_NAMES") or []
+        for m in range(int(s["N_CHEM"])):
+            f[names[m] if m < len(names) else f"chem{m}"] = pr[int(s["CHEM_OFFSET"]) + m]
     base = dict(rho=pr[0], vx=pr[1], vy=pr[2], vz=pr[3], p=pr[4])
     if s["PHYSICS"] == "rmhd":
         base.update(Bx=pr[5], By=pr[6], Bz=pr[7], psi=pr[8])
@@ -174,7 +179,7 @@ def run(s, log=print):
             d.apply_bcs(rng)
             d.step(dt)
             d.apply_bcs(rng)
-            d.two_temperature(dt)
+            d.post_step_sources(dt)  # cooling/heating, two-temperature, chemistry (:324-326)
             t += dt
             step += 1
             if step % int(s["PRINT_EVERY"]) == 0 or abs(t - T_END) < 1e-14:

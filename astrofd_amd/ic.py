@@ -52,6 +52,9 @@ def init_block(s):
     if int(s.get("N_THERMO", 0)) > 0:
         pr[int(s["THERMO_OFFSET"])] = float(s.get("TE_AMB", 0.0) or 0.0)
         pr[int(s["THERMO_OFFSET"]) + 1] = float(s.get("TI_AMB", 0.0) or 0.0)
+    if int(s.get("N_CHEM", 0)) > 0:
+        for m, key in enumerate(("CHEM_X_HII_AMB", "CHEM_X_HEII_AMB", "CHEM_X_HEIII_AMB")):
+            pr[int(s["CHEM_OFFSET"]) + m] = float(s.get(key, 0.0))
     return pr
 
 

@@ -18,7 +18,9 @@ reports the relative L1 and max error of rho, p and W after checkpoint steps.
 
 Prints ONE JSON line.  Also: the dominant kernel's roofline from HIP events on
 the library's stream over the timed steps, the solver counters of one extra
-step, and the CPU oracle's rate on a bounded sample on this host's cores.
+step, the fused post-step source kernel (cooling, two-temperature, chemistry)
+on the same grid, and the CPU oracle's rate on a bounded sample on this host's
+cores.
 """
 import argparse
 import json
@@ -87,7 +89,7 @@ def make_stepper(d, s, rng):
         d.apply_bcs(rng)
         d.step(dt)
         d.apply_bcs(rng)
-        d.two_temperature(dt)
+        d.post_step_sources(dt)  # none in the RMHD jet workload: returns without a launch
     return one_step
 
 
@@ -287,6 +289,48 @@ def cpu_baseline(s, n=64, steps=2, n1=32):
                       f"and 1 step at {n1}^3 on 1 thread ({el_one:.2f} s): C oracle, FP64, OpenMP"}
 
 
+SOURCE_BYTES_PER_CELL = 7 * 8 + 6 * 8  # read rho, p, Te, Ti, 3 ion fractions; write all but rho
+
+
+def sources_roofline(s, pr0, launches=20):
+    """The fused post-step source kernel (cooling/heating + two-temperature + H/He chemistry,
+    solvers/srhd3d_mpi_muscl.py:324-326) on the workload's grid and state, with Te, Ti and the ion
+    fractions appended as passives; HIP-event time per launch and its algorithmic HBM bandwidth
+    (SOURCE_BYTES_PER_CELL per interior cell)."""
+    from astrofd_amd.domain import Domain
+    nb = 9 if s["PHYSICS"] == "rmhd" else 5
+    t = dict(s, N_TRACERS=0, N_THERMO=2, N_CHEM=3, N_PASSIVE=5, PASSIVE_OFFSET=nb, THERMO_OFFSET=nb,
+             CHEM_OFFSET=nb + 2, TWO_TEMPERATURE=True, TEI_TAU=0.5, COOLING_ENABLED=True, COOLING_LAMBDA=0.1,
+             HEATING_RATE=1e-4, CHEMISTRY_ENABLED=True, CHEM_TUNIT_K=1e6, CHEM_RATE_SCALE=1e8,
+             CHEM_ENERGY_SCALE=1e-6)
+    pr = np.zeros((nb + 5,) + pr0.shape[1:])
+    pr[:nb] = pr0[:nb]
+    pr[nb] = pr[nb + 1] = pr0[4] / pr0[0]
+    pr[nb + 2], pr[nb + 3] = 0.1, 0.05
+    d = Domain.from_settings(t)
+    try:
+        d.set_prim(pr)
+        del pr
+        for _ in range(2):
+            d.post_step_sources(1e-3)
+        d.set_instrumentation(stats=False, timing=True)
+        d.reset_stats()
+        for _ in range(launches):
+            d.post_step_sources(1e-3)
+        sync(d)
+        ms = d.stats()["ms_other"] / launches
+        d.set_instrumentation(stats=False, timing=False)
+        if d.count_nonfinite():
+            raise RuntimeError("post-step sources produced non-finite values")
+    finally:
+        d.close()
+    nbytes = SOURCE_BYTES_PER_CELL * s["NX"] * s["NY"] * s["NZ"]
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    return {"kernel": "post_step_sources", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "avg_ms": round(ms, 4),
+            "bytes_per_launch": int(nbytes), "rocprof_name": "k_sources(SourceArgs)"}
+
+
 def kernel_ms(k, timing_steps):
     return {"face_x": round(k["ms_face"][0] / timing_steps, 3), "face_y": round(k["ms_face"][1] / timing_steps, 3),
             "face_z": round(k["ms_face"][2] / timing_steps, 3),
@@ -322,6 +366,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the FP64 plain run")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the lockstep accuracy pass")
+    ap.add_argument("--no-sources", action="store_true", help="skip the post-step source kernel measurement")
+    ap.add_argument("--sources-only", action="store_true",
+                    help="measure only the post-step source kernel and print its JSON line (profiling)")
     ap.add_argument("--accuracy-steps", default="1,2,5,10")
     ap.add_argument("--evolve", action="store_true",
                     help="time consecutive steps of one run instead of steps from the seeded state")
@@ -344,6 +391,9 @@ def main():
     pr0 = ic.initial_state(s)
     log(f"[bench] initial state {pr0.shape} in {time.perf_counter() - t0:.1f}s")
     zu = s["NX"] * s["NY"] * s["NZ"] * a.steps
+    if a.sources_only:
+        print(json.dumps({"workload": a.workload, "post_step_sources": sources_roofline(s, pr0)}), flush=True)
+        return
 
     mlp = None
     if s["PHYSICS"] == "rmhd" and os.path.exists(MLP_WEIGHTS):
@@ -426,6 +476,9 @@ def main():
             res["consecutive_steps"] = {"steps": a.evolve_steps, "fp64_plain": round(zu_e / ep["elapsed"], 1),
                                         "mixed": round(zu_e / em["elapsed"], 1),
                                         "mixed_c2p": c2p_report(em["stats"])}
+    if not a.no_sources:
+        log("[bench] post-step sources kernel")
+        res["post_step_sources"] = sources_roofline(s, pr0)
     if not a.no_cpu_baseline:
         log("[bench] CPU oracle sample")
         res["cpu_baseline"] = cpu_baseline(s)

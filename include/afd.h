@@ -86,6 +86,9 @@ typedef struct afd_nozzle {
     int32_t b_init;           /* 0 none, 1 poloidal (Bx = B0 s), 2 toroidal (By = B0 s) */
     double b0;
     double b_amb[3];          /* additive ambient field added to the nozzle B (zeros: reference) */
+    int32_t chemistry;        /* CHEMISTRY_ENABLED: write the nozzle ion fractions */
+    int32_t chem_offset;      /* CHEM_OFFSET */
+    double chem_nozzle[3];    /* CHEM_X_HII_NOZZLE, CHEM_X_HEII_NOZZLE, CHEM_X_HEIII_NOZZLE */
 } afd_nozzle;
 
 typedef struct afd_stats {
@@ -146,8 +149,42 @@ int afd_step(afd_ctx *ctx, double dt);
  * core/rmhd_core.py:748 (compute_rhs_rmhd), :1279 (ppm), :1439 (weno). */
 int afd_compute_rhs(afd_ctx *ctx, double *rhs_host, int64_t n);
 
-/* core/source_terms.py:30-55 (apply_two_temperature) on the current state. */
+/* core/source_terms.py:30-55 (apply_two_temperature) on the current state's interior;
+ * a no-op unless the context's two_temperature is set and tei_tau > 0. */
 int afd_two_temperature(afd_ctx *ctx, double dt);
+
+/* core/source_terms.py:5-28 (apply_cooling_heating) on the current state's interior:
+ * p <- min(max(p + dt (heat - cool rho p), 1e-12), p_cap), with cool = COOLING_LAMBDA,
+ * heat = HEATING_RATE and p_cap = the settings' P_MAX (the reference's default is 1.0).
+ * A no-op when cool and heat are both zero. */
+int afd_cooling_heating(afd_ctx *ctx, double dt, double cool, double heat, double p_cap);
+
+/* Constants of core/chemistry.py:59-68, read from the settings by the caller. */
+typedef struct afd_chem {
+    int32_t offset;           /* CHEM_OFFSET: xHII, xHeII, xHeIII at offset, offset+1, offset+2 (>= nb) */
+    double X, Y;              /* CHEM_X, CHEM_Y */
+    double tunit_k;           /* CHEM_TUNIT_K */
+    double rate_scale, energy_scale;
+    double tmin_k;            /* CHEM_TMIN_K */
+    double gamma;             /* GAMMA of the settings (default 5/3) */
+} afd_chem;
+
+/* core/chemistry.py:43-139 (apply_ion_chemistry) on the current state's interior: the IMEX
+ * update of the H/He ion fractions and its pressure coupling.  A no-op when nv <= offset + 2;
+ * the caller checks CHEMISTRY_ENABLED and N_CHEM >= 3 (core/chemistry.py:48-53). */
+int afd_ion_chemistry(afd_ctx *ctx, double dt, const afd_chem *chem);
+
+typedef struct afd_sources {
+    int32_t cooling;          /* COOLING_ENABLED */
+    double cool, heat, p_cap; /* as afd_cooling_heating */
+    int32_t chemistry;        /* CHEMISTRY_ENABLED and N_CHEM >= 3 */
+    afd_chem chem;
+} afd_sources;
+
+/* The driver's post-step sources, solvers/srhd3d_mpi_muscl.py:324-326 (apply_cooling_heating,
+ * apply_two_temperature, apply_ion_chemistry), in one pass over the interior that applies them
+ * to each cell in that order.  Two-temperature follows the context's parameters. */
+int afd_post_step_sources(afd_ctx *ctx, double dt, const afd_sources *sources);
 
 /* Non-finite values in the current state (the driver's NaN check,
  * solvers/srhd3d_mpi_muscl.py:337-343). */

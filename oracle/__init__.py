@@ -47,7 +47,13 @@ class Nozzle(ctypes.Structure):
         ("two_temperature", ctypes.c_int), ("thermo_offset", ctypes.c_int),
         ("te_nozzle", ctypes.c_double), ("ti_nozzle", ctypes.c_double),
         ("b_init", ctypes.c_int), ("b0", ctypes.c_double), ("b_amb", ctypes.c_double * 3),
+        ("chemistry", ctypes.c_int), ("chem_offset", ctypes.c_int), ("chem_nozzle", ctypes.c_double * 3),
     ]
+
+
+class Chem(ctypes.Structure):
+    _fields_ = [("offset", ctypes.c_int)] + [(n, ctypes.c_double) for n in
+                                            ("X", "Y", "tunit", "rate_scale", "energy_scale", "tmin", "gamma")]
 
 
 def build():
@@ -80,6 +86,8 @@ def lib():
             "orc_apply_periodic_yz": [P, _D], "orc_apply_outflow_right_x": [P, _D],
             "orc_apply_outflow_yz": [P, _D, ctypes.c_int, ctypes.c_int],
             "orc_apply_two_temperature": [P, _D, ctypes.c_double, ctypes.c_double, ctypes.c_int],
+            "orc_apply_cooling_heating": [P, _D] + [ctypes.c_double] * 4,
+            "orc_apply_ion_chemistry": [P, _D, ctypes.c_double, ctypes.POINTER(Chem)],
         }
         for name, args in sig.items():
             getattr(L, name).argtypes = args
@@ -285,6 +293,10 @@ def apply_nozzle_left_x(P, pr, dx, dy, dz, ny_loc, nz_loc, y0, z0, cfg, draws=No
                ti_nozzle=float(cfg.get("TI_NOZZLE", cfg.get("TI_AMB", 0.0)) or 0.0),
                b_init=B_INITS.get(cfg.get("B_INIT", "none"), 0), b0=float(cfg.get("B0", 0.0)))
     N.b_amb[0], N.b_amb[1], N.b_amb[2] = b_amb
+    if cfg.get("CHEMISTRY_ENABLED", False):  # core/nozzle.py:77-82
+        N.chemistry, N.chem_offset = 1, int(cfg.get("CHEM_OFFSET", 0))
+        for m, key in enumerate(("CHEM_X_HII_NOZZLE", "CHEM_X_HEII_NOZZLE", "CHEM_X_HEIII_NOZZLE")):
+            N.chem_nozzle[m] = float(cfg.get(key, 0.0))
     d = _c(draws if draws is not None else np.zeros(0))
     return lib().orc_apply_nozzle_left_x(ctypes.byref(Q), ctypes.byref(N), _p(pr), _p(d), d.size)
 
@@ -292,6 +304,33 @@ def apply_nozzle_left_x(P, pr, dx, dy, dz, ny_loc, nz_loc, y0, z0, cfg, draws=No
 def apply_two_temperature(P, pr, dt, tei_tau, thermo_offset):
     Q = with_shape(P, pr)
     lib().orc_apply_two_temperature(ctypes.byref(Q), _p(pr), dt, tei_tau, thermo_offset)
+
+
+def apply_cooling_heating(P, pr, dt, cfg):
+    """core/source_terms.py:5-28 on `pr` in place (the cfg gates and defaults included)."""
+    if not cfg.get("COOLING_ENABLED", False):
+        return
+    Q = with_shape(P, pr)
+    lib().orc_apply_cooling_heating(ctypes.byref(Q), _p(pr), dt, float(cfg.get("COOLING_LAMBDA", 0.0)),
+                                    float(cfg.get("HEATING_RATE", 0.0)), float(cfg.get("P_MAX", 1.0)))
+
+
+def chem_from_cfg(cfg):
+    """The constants core/chemistry.py:59-68 reads from the settings."""
+    return Chem(offset=int(cfg.get("CHEM_OFFSET", 0)), X=float(cfg.get("CHEM_X", 0.76)),
+                Y=float(cfg.get("CHEM_Y", 0.24)), tunit=float(cfg.get("CHEM_TUNIT_K", 1.0e4)),
+                rate_scale=float(cfg.get("CHEM_RATE_SCALE", 1.0)),
+                energy_scale=float(cfg.get("CHEM_ENERGY_SCALE", 1.0)), tmin=float(cfg.get("CHEM_TMIN_K", 1.0)),
+                gamma=float(cfg.get("GAMMA", 5.0 / 3.0)))
+
+
+def apply_ion_chemistry(P, pr, dt, cfg):
+    """core/chemistry.py:43-139 on `pr` in place (the cfg gates and defaults included)."""
+    if not cfg.get("CHEMISTRY_ENABLED", False) or int(cfg.get("N_CHEM", 0)) < 3:
+        return
+    Q = with_shape(P, pr)
+    c = chem_from_cfg(cfg)
+    lib().orc_apply_ion_chemistry(ctypes.byref(Q), _p(pr), dt, ctypes.byref(c))
 
 
 def num_threads():

@@ -1261,6 +1261,8 @@ int64_t orc_apply_nozzle_left_x(const orc_params *P, const orc_nozzle *N, double
                 PR(2, g, j, k) = vy;
                 PR(3, g, j, k) = vz;
                 PR(4, g, j, k) = p;
+                if (P->nv >= 15)  /* core/nozzle.py:60-61 */
+                    for (int v = 5; v < 15; ++v) PR(v, g, j, k) = 0.0;
                 for (int t = 0; t < N->n_tracers; ++t) {
                     double tn = N->tracer_nozzle[t], ta = N->tracer_amb[t];
                     PR(N->tracer_offset + t, g, j, k) = ta + (tn - ta) * s;
@@ -1269,6 +1271,8 @@ int64_t orc_apply_nozzle_left_x(const orc_params *P, const orc_nozzle *N, double
                     PR(N->thermo_offset, g, j, k) = N->te_nozzle;
                     PR(N->thermo_offset + 1, g, j, k) = N->ti_nozzle;
                 }
+                if (N->chemistry && P->nv > N->chem_offset + 2)  /* core/nozzle.py:77-82 */
+                    for (int m = 0; m < 3; ++m) PR(N->chem_offset + m, g, j, k) = N->chem_nozzle[m];
                 if (P->physics == 1) {
                     double Bx = 0.0, By = 0.0, Bz = 0.0;
                     if (N->b_init == 1) Bx = N->b0 * s;
@@ -1297,5 +1301,74 @@ void orc_apply_two_temperature(const orc_params *P, double *pr, double dt, doubl
                 Ti += dt * (T - Ti) / tau;
                 PR(off, i, j, k) = Te;
                 PR(off + 1, i, j, k) = Ti;
+            }
+}
+
+/* core/source_terms.py:5-28 (apply_cooling_heating), interior cells */
+void orc_apply_cooling_heating(const orc_params *P, double *pr, double dt, double cool, double heat, double pmax) {
+    if (cool == 0.0 && heat == 0.0) return;
+    const int ng = P->ng;
+    for (int i = ng; i < NXF - ng; ++i)
+        for (int j = ng; j < NYF - ng; ++j)
+            for (int k = ng; k < NZF - ng; ++k) {
+                double rho = PR(0, i, j, k), p = PR(4, i, j, k);
+                double dp = dt * (heat - cool * rho * p);
+                double p_new = p + dp;
+                if (p_new < 1e-12) p_new = 1e-12;
+                if (p_new > pmax) p_new = pmax;
+                PR(4, i, j, k) = p_new;
+            }
+}
+
+/* core/chemistry.py:8-40 (rates; T in K); Python's "a ** b" is libm pow */
+#define CHEM_SMALL 1e-20
+static double chem_clamp(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+static double rate_ci(double a, double e, double T) {
+    return a * sqrt(T) * exp(-e / pymax(T, 1.0)) / (1.0 + sqrt(T / 1.0e5));
+}
+
+/* core/chemistry.py:43-139 (apply_ion_chemistry), interior cells */
+void orc_apply_ion_chemistry(const orc_params *P, double *pr, double dt, const orc_chem *c) {
+    const int ng = P->ng, off = c->offset;
+    if (P->nv <= off + 2) return;
+    const double e_h = 157809.1, e_he1 = 285335.4, e_he2 = 631515.0;
+    for (int i = ng; i < NXF - ng; ++i)
+        for (int j = ng; j < NYF - ng; ++j)
+            for (int k = ng; k < NZF - ng; ++k) {
+                double rho = PR(0, i, j, k), p = PR(4, i, j, k);
+                if (rho <= 0.0 || p <= 0.0) continue;
+                double T = (p / pymax(rho, CHEM_SMALL)) * c->tunit;
+                if (T < c->tmin) T = c->tmin;
+                double nH = c->X * rho, nHe = c->Y * rho / 4.0;
+                double xHII = chem_clamp(PR(off, i, j, k), 0.0, 1.0);
+                double xHeII = chem_clamp(PR(off + 1, i, j, k), 0.0, 1.0);
+                double xHeIII = chem_clamp(PR(off + 2, i, j, k), 0.0, 1.0);
+                xHeII = pymin(xHeII, 1.0 - xHeIII);
+                double xHeI = pymax(0.0, 1.0 - xHeII - xHeIII);
+                double xHI = 1.0 - xHII;
+                double ne = nH * xHII + nHe * (xHeII + 2.0 * xHeIII);
+                ne = pymax(ne, CHEM_SMALL);
+                double C_H = c->rate_scale * rate_ci(5.85e-11, 157809.1, T);
+                double C_HeI = c->rate_scale * rate_ci(2.38e-11, 285335.4, T);
+                double C_HeII = c->rate_scale * rate_ci(5.68e-12, 631515.0, T);
+                double A_H = c->rate_scale * (2.59e-13 * libm_pow(T / 1.0e4, -0.7));
+                double A_HeII = c->rate_scale * (1.50e-10 * libm_pow(T / 1.0e4, -0.6353));
+                double A_HeIII = c->rate_scale * (3.36e-10 * libm_pow(T / 1.0e4, -0.5));
+                double xHII_new = (xHII + dt * ne * C_H * xHI) / (1.0 + dt * ne * A_H);
+                double xHeIII_new = (xHeIII + dt * ne * C_HeII * xHeII) / (1.0 + dt * ne * A_HeIII);
+                xHeIII_new = chem_clamp(xHeIII_new, 0.0, 1.0);
+                double xHeII_new = (xHeII + dt * ne * (C_HeI * xHeI + A_HeIII * xHeIII_new)) /
+                                   (1.0 + dt * ne * (A_HeII + C_HeII));
+                xHII_new = chem_clamp(xHII_new, 0.0, 1.0);
+                xHeII_new = chem_clamp(xHeII_new, 0.0, 1.0 - xHeIII_new);
+                double dH = (xHII_new - xHII), dHeII = (xHeII_new - xHeII), dHeIII = (xHeIII_new - xHeIII);
+                double dE = c->energy_scale * (nH * e_h * dH + nHe * e_he1 * dHeII + nHe * e_he2 * dHeIII);
+                double dp = -(c->gamma - 1.0) * dE / pymax(c->tunit, CHEM_SMALL);
+                double p_new = p + dp;
+                if (p_new < CHEM_SMALL) p_new = CHEM_SMALL;
+                PR(4, i, j, k) = p_new;
+                PR(off, i, j, k) = xHII_new;
+                PR(off + 1, i, j, k) = xHeII_new;
+                PR(off + 2, i, j, k) = xHeIII_new;
             }
 }

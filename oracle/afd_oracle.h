@@ -87,6 +87,8 @@ typedef struct orc_nozzle {
     int b_init;               /* 0 none, 1 poloidal, 2 toroidal */
     double b0;
     double b_amb[3];          /* additive B_AMB (zeros keep reference behaviour) */
+    int chemistry, chem_offset;
+    double chem_nozzle[3];    /* CHEM_X_HII/HEII/HEIII_NOZZLE */
 } orc_nozzle;
 
 double orc_nozzle_weight(double r, double radius, double shear_thick, int profile);
@@ -95,6 +97,12 @@ double orc_nozzle_weight(double r, double radius, double shear_thick, int profil
 int64_t orc_apply_nozzle_left_x(const orc_params *P, const orc_nozzle *N, double *pr,
                                 const double *draws, int64_t ndraws);
 void orc_apply_two_temperature(const orc_params *P, double *pr, double dt, double tei_tau, int thermo_offset);
+void orc_apply_cooling_heating(const orc_params *P, double *pr, double dt, double cool, double heat, double pmax);
+typedef struct orc_chem {
+    int offset;
+    double X, Y, tunit, rate_scale, energy_scale, tmin, gamma;
+} orc_chem;
+void orc_apply_ion_chemistry(const orc_params *P, double *pr, double dt, const orc_chem *chem);
 
 /* number of OpenMP threads the whole-array operators use (1 when built without OpenMP) */
 int orc_num_threads(void);

@@ -48,7 +48,7 @@ def _install_numba_stub():
 
 _install_numba_stub()
 sys.path.insert(0, REF)
-from core import srhd_core, rmhd_core, boundary, nozzle, source_terms  # noqa: E402
+from core import srhd_core, rmhd_core, boundary, nozzle, source_terms, chemistry  # noqa: E402
 
 
 def _save(name, **arrays):
@@ -61,7 +61,7 @@ def _save(name, **arrays):
 # Seeded primitive states
 # ---------------------------------------------------------------------------
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
-from tests.golden_states import rmhd_state, smooth_state, srhd_state  # noqa: E402
+from tests.golden_states import rmhd_state, smooth_state, source_state, srhd_state  # noqa: E402
 
 
 SRHD_BASE = dict(GAMMA=5.0 / 3.0, P_MAX=2.0, V_MAX=0.9999, N_PASSIVE=0, PASSIVE_OFFSET=5)
@@ -394,6 +394,90 @@ def gen_bc():
 
 
 # ---------------------------------------------------------------------------
+# Post-step sources (core/source_terms.py, core/chemistry.py) and the nozzle's
+# chemistry fill (core/nozzle.py:77-82)
+# ---------------------------------------------------------------------------
+COOLING_CASES = [
+    # dt, settings (P_MAX absent: the reference's default 1.0)
+    (0.05, dict(COOLING_ENABLED=True, COOLING_LAMBDA=5.0, HEATING_RATE=0.0, P_MAX=2.0)),
+    (0.5, dict(COOLING_ENABLED=True, COOLING_LAMBDA=0.0, HEATING_RATE=0.3)),
+    (0.1, dict(COOLING_ENABLED=True, COOLING_LAMBDA=50.0, HEATING_RATE=0.01, P_MAX=2.0)),
+    (0.1, dict(COOLING_ENABLED=True, COOLING_LAMBDA=0.0, HEATING_RATE=0.0)),
+    (0.1, dict(COOLING_ENABLED=False, COOLING_LAMBDA=5.0, HEATING_RATE=0.2)),
+]
+CHEM_CASES = [
+    (0.01, dict()),
+    (0.05, dict(CHEM_TUNIT_K=1.0e6, CHEM_RATE_SCALE=1.0e9, CHEM_ENERGY_SCALE=1.0e-7, CHEM_TMIN_K=5.0e3,
+                GAMMA=4.0 / 3.0)),
+    (0.1, dict(CHEM_TUNIT_K=1.0e7, CHEM_RATE_SCALE=1.0e12, CHEM_ENERGY_SCALE=1.0e-4)),
+    (1e-3, dict(CHEM_X=0.7, CHEM_Y=0.3, CHEM_TUNIT_K=1.0e5, CHEM_RATE_SCALE=1.0e10, CHEM_ENERGY_SCALE=1.0e-6)),
+    # strong coupling: the pressure floor (1e-20) and the clamps of xHII and xHeIII at 1 act
+    (0.05, dict(CHEM_TUNIT_K=1.0e8, CHEM_RATE_SCALE=1.0e10, CHEM_ENERGY_SCALE=10.0)),
+]
+
+
+def gen_sources():
+    ng = 2
+    shape = (4 + 2 * ng, 5 + 2 * ng, 6 + 2 * ng)
+    arrs = {}
+    # cooling/heating: rho in (0.01, 2), p in (1e-6, 1.5)
+    for c, (dt, cfg) in enumerate(COOLING_CASES):
+        rng = np.random.default_rng(700 + c)
+        pr = np.zeros((5,) + shape)
+        pr[0] = rng.uniform(0.01, 2.0, shape)
+        pr[4] = 10.0 ** rng.uniform(-6, np.log10(1.5), shape)
+        arrs[f"cool_in_{c}"] = pr.copy()
+        source_terms.apply_cooling_heating(pr, dt, dict(cfg, NG=ng))
+        arrs[f"cool_out_{c}"] = pr[4]  # the only variable it writes
+    # ion chemistry on hydro (nv = 8) and RMHD (nv = 12) layouts
+    for c, (dt, cfg) in enumerate(CHEM_CASES):
+        nvb = 9 if c % 2 else 5
+        pr = source_state(shape, 720 + c, nvb + 3, chem_off=nvb)
+        full = dict(cfg, CHEMISTRY_ENABLED=True, N_CHEM=3, CHEM_OFFSET=nvb, NG=ng)
+        arrs[f"chem_in_{c}"] = pr.copy()
+        chemistry.apply_ion_chemistry(pr, dt, full)
+        arrs[f"chem_out_{c}"] = pr[[4, nvb, nvb + 1, nvb + 2]]  # p and the fractions
+    # the driver's post-step order (solvers/srhd3d_mpi_muscl.py:324-326): cooling, two-temperature,
+    # chemistry, on one hydro block with a tracer, Te/Ti and the ion fractions
+    cfg = dict(NG=ng, COOLING_ENABLED=True, COOLING_LAMBDA=2.0, HEATING_RATE=0.05, P_MAX=50.0,
+               TWO_TEMPERATURE=True, N_TRACERS=1, THERMO_OFFSET=6, TEI_TAU=0.2,
+               CHEMISTRY_ENABLED=True, N_CHEM=3, CHEM_OFFSET=8, CHEM_TUNIT_K=1.0e6, CHEM_RATE_SCALE=1.0e9,
+               CHEM_ENERGY_SCALE=1.0e-8)
+    pr = source_state(shape, 740, 11, chem_off=8, bad=False)
+    arrs["chain_in"] = pr.copy()
+    source_terms.apply_cooling_heating(pr, 0.02, cfg)
+    source_terms.apply_two_temperature(pr, 0.02, cfg)
+    chemistry.apply_ion_chemistry(pr, 0.02, cfg)
+    arrs["chain_out"] = pr[[4, 6, 7, 8, 9, 10]]  # p, Te, Ti and the fractions
+    # nozzle fill with chemistry fractions; the 16-variable layouts also take the pr[5:15] = 0 branch
+    NX, NY, NZ = 2, 6, 5
+    nshape = (NX + 2 * ng, NY + 2 * ng, NZ + 2 * ng)
+    noz_cases = [
+        # physics, n_tracers, two_temperature, chem_offset, nv, b_init
+        ("hydro", 1, True, 8, 11, "none"),
+        ("rmhd", 2, True, 13, 16, "poloidal"),
+        ("hydro", 3, True, 13, 16, "none"),
+    ]
+    for c, (phys, ntr, tt, coff, nv, binit) in enumerate(noz_cases):
+        nvb = 9 if phys == "rmhd" else 5
+        cfg = dict(NG=ng, JET_RADIUS=0.3, SHEAR_THICK=0.02, NOZZLE_PROFILE="top_hat", GAMMA_JET=7.0888121,
+                   ETA_RHO=0.01, P_EQ=0.01, RHO_AMB=1.0, VX_AMB=0.01, VY_AMB=-0.02, VZ_AMB=0.03,
+                   NOZZLE_PERTURB="random", TURB_VAMP=0.02, TURB_PAMP=0.1, N_TRACERS=ntr, TRACER_OFFSET=nvb,
+                   TRACER_NOZZLE_VALUES=[1.0, 0.5, 0.2][:ntr], TRACER_AMB_VALUES=[0.0, 0.25, 0.1][:ntr],
+                   TWO_TEMPERATURE=tt, THERMO_OFFSET=nvb + ntr, TE_NOZZLE=0.7, TI_NOZZLE=0.9,
+                   CHEMISTRY_ENABLED=True, CHEM_OFFSET=coff, CHEM_X_HII_NOZZLE=0.9, CHEM_X_HEII_NOZZLE=0.3,
+                   CHEM_X_HEIII_NOZZLE=0.05, PHYSICS=phys, B_INIT=binit, B0=0.5)
+        pr = source_state(nshape, 760 + c, nv, chem_off=coff, bad=False)
+        arrs[f"noz_in_{c}"] = pr.copy()
+        rng = np.random.default_rng(1234)
+        nozzle.apply_nozzle_left_x(pr, 1.0 / NX, 1.0 / NY, 1.0 / NZ, NY, NZ, 0.45, 0.55, rng, cfg)
+        arrs[f"noz_out_{c}"] = pr[:, :ng]  # the fill writes the -x ghost planes only
+    _save("sources", cool_cases=np.array([repr(c) for c in COOLING_CASES]),
+          chem_cases=np.array([repr(c) for c in CHEM_CASES]),
+          noz_cases=np.array(["/".join(map(str, c)) for c in noz_cases]), ng=np.array(ng), **arrs)
+
+
+# ---------------------------------------------------------------------------
 # Driver loop (restates the loop order of solvers/srhd3d_mpi_muscl.py:276-326, one rank)
 # ---------------------------------------------------------------------------
 def run_driver(s, pr, nsteps, rng):
@@ -419,7 +503,9 @@ def run_driver(s, pr, nsteps, rng):
         boundary.apply_periodic_yz(pr, ng)
         boundary.apply_outflow_right_x(pr, ng)
         nozzle.apply_nozzle_left_x(pr, dx, dy, dz, s["NY"], s["NZ"], y0, z0, rng, s)
+        pr = source_terms.apply_cooling_heating(pr, dt, s)
         pr = source_terms.apply_two_temperature(pr, dt, s)
+        pr = chemistry.apply_ion_chemistry(pr, dt, s)
         dts.append(dt)
         print(f"[golden] driver step {step+1}/{nsteps} dt={dt:.4e} amax={amax:.4f}", flush=True)
     return pr, np.array(dts)
@@ -445,6 +531,9 @@ def init_block(s, noise_seed):
     pr[0] = s["RHO_AMB"]; pr[4] = s["P_AMB"]
     if s["TWO_TEMPERATURE"]:
         pr[s["THERMO_OFFSET"]] = s["TE_AMB"]; pr[s["THERMO_OFFSET"] + 1] = s["TI_AMB"]
+    if s.get("CHEMISTRY_ENABLED", False):
+        for m, key in enumerate(("CHEM_X_HII_AMB", "CHEM_X_HEII_AMB", "CHEM_X_HEIII_AMB")):
+            pr[s["CHEM_OFFSET"] + m] = s[key]
     if noise_seed is not None:
         noise = np.random.default_rng(noise_seed).uniform(-0.01, 0.01, (s["NX"], s["NY"], s["NZ"]))
         pr[0, ng:-ng, ng:-ng, ng:-ng] = s["RHO_AMB"] * (1.0 + noise)
@@ -479,6 +568,28 @@ def gen_driver_small():
     _save("driver_srhd_10", pr0=pr0, out=out, dts=dts, settings=np.array(repr(s)))
 
 
+def gen_driver_sources():
+    # SRHD MUSCL-MC + HLLC + SSPRK3 jet with every post-step source on: cooling/heating,
+    # two-temperature and H/He chemistry (rates and energy coupling scaled up so that all of them
+    # move the state within 3 steps), a tracer, and the nozzle's ion fractions.
+    s = driver_settings(NX=10, NY=9, NZ=8, NG=2, RECON="muscl", LIMITER="mc", RIEMANN="hllc", RK_ORDER=3,
+                        NOZZLE_PERTURB="random", TURB_PAMP=0.05, SHEAR_THICK=0.02, JET_RADIUS=0.3,
+                        N_TRACERS=1, TRACER_NOZZLE_VALUES=[1.0], TRACER_AMB_VALUES=[0.0], TRACER_OFFSET=5,
+                        TWO_TEMPERATURE=True, TEI_TAU=0.1, THERMO_OFFSET=6, TE_AMB=0.01, TI_AMB=0.01,
+                        TE_NOZZLE=0.5, TI_NOZZLE=0.2,
+                        COOLING_ENABLED=True, COOLING_LAMBDA=0.5, HEATING_RATE=0.005,
+                        CHEMISTRY_ENABLED=True, N_CHEM=3, CHEM_OFFSET=8, CHEM_TUNIT_K=1.0e6,
+                        CHEM_RATE_SCALE=1.0e8, CHEM_ENERGY_SCALE=1.0e-3, CHEM_X_HII_AMB=0.1,
+                        CHEM_X_HEII_AMB=0.05, CHEM_X_HEIII_AMB=0.0, CHEM_X_HII_NOZZLE=0.9,
+                        CHEM_X_HEII_NOZZLE=0.3, CHEM_X_HEIII_NOZZLE=0.05, N_PASSIVE=6, PASSIVE_OFFSET=5)
+    pr0 = init_block(s, 4242)
+    t0 = time.time()
+    out, dts = run_driver(s, pr0.copy(), 3, np.random.default_rng(1234))
+    print(f"[golden] srhd sources driver {time.time()-t0:.1f}s: xHII [{out[8].min():.3e}, {out[8].max():.3e}]",
+          flush=True)
+    _save("driver_sources_10", pr0=pr0, out=out, dts=dts, settings=np.array(repr(s)))
+
+
 def gen_driver_32():
     # The 32^3 SRHD jet: MUSCL-minmod + HLLE + SSPRK2, CFL 0.4, 20 steps, +-1% density noise
     # (default_rng(12345)), nozzle radius 4 cells, W_jet = 7.0888; reference periodic y/z boundaries.
@@ -490,7 +601,8 @@ def gen_driver_32():
     _save("driver_srhd_32", pr0=pr0, out=out, dts=dts, settings=np.array(repr(s)))
 
 
-GENS = dict(pointwise=gen_pointwise, rhs=gen_rhs, steps=gen_steps, bc=gen_bc, driver_small=gen_driver_small)
+GENS = dict(pointwise=gen_pointwise, rhs=gen_rhs, steps=gen_steps, bc=gen_bc, sources=gen_sources,
+            driver_small=gen_driver_small, driver_sources=gen_driver_sources)
 
 
 def main():

@@ -73,3 +73,25 @@ def smooth_state(shape, seed, physics="hydro", vamp=0.2, npass=0):
     for t in range(npass):
         pr[nb + t] = 0.5 + field(0.4)
     return pr
+
+
+def source_state(shape, seed, nv, chem_off=None, bad=True):
+    """rho and p log-uniform over wide ranges (T = p / rho * CHEM_TUNIT_K from ~10 K to ~1e12 K),
+    velocities and fields small, ion fractions in [-0.2, 1.2] so the clamps act, and (bad=True)
+    a few cells with rho <= 0 or p <= 0, which chemistry skips.  The other variables are zero
+    (the sources do not read them), which keeps the fixture small."""
+    rng = np.random.default_rng(seed)
+    pr = np.zeros((nv,) + shape)
+    pr[0] = 10.0 ** rng.uniform(-3, 1, shape)
+    pr[4] = 10.0 ** rng.uniform(-4, 3, shape)
+    if chem_off is not None:
+        pr[chem_off:chem_off + 3] = rng.uniform(-0.2, 1.2, (3,) + shape)
+    if bad:
+        flat = pr.reshape(nv, -1)
+        n = flat.shape[1]
+        cells = rng.choice(n, 12, replace=False)
+        flat[0, cells[:4]] = [0.0, -1.0, 0.0, -1e-3]
+        flat[4, cells[4:8]] = [0.0, -2.0, 0.0, -1e-6]
+        flat[0, cells[8:10]] = 0.0
+        flat[4, cells[10:12]] = 0.0
+    return pr

@@ -158,14 +158,14 @@ def run_gpu_driver(s, pr0, nsteps, seed=1234):
             d.apply_bcs(rng)
             d.step(dt)
             d.apply_bcs(rng)
-            d.two_temperature(dt)
+            d.post_step_sources(dt)
             dts.append(dt)
         return d.get_prim(), np.array(dts)
     finally:
         d.close()
 
 
-@pytest.mark.parametrize("name", ["driver_srhd_10", "driver_rmhd_8", "driver_srhd_32"])
+@pytest.mark.parametrize("name", ["driver_srhd_10", "driver_rmhd_8", "driver_sources_10", "driver_srhd_32"])
 def test_driver_golden(name):
     path = os.path.join(G, name + ".npz")
     if not os.path.exists(path):

@@ -165,7 +165,7 @@ def test_bc_nozzle_two_temperature():
 
 
 # ---------------------------------------------------------------- driver loops
-@pytest.mark.parametrize("name", ["driver_srhd_10", "driver_rmhd_8", "driver_srhd_32"])
+@pytest.mark.parametrize("name", ["driver_srhd_10", "driver_rmhd_8", "driver_sources_10", "driver_srhd_32"])
 def test_driver_loop(name):
     path = os.path.join(G, name + ".npz")
     if not os.path.exists(path):
@@ -176,3 +176,89 @@ def test_driver_loop(name):
     out, dts = oracle_loop.run(s, f["pr0"].copy(), len(f["dts"]), b_amb=b_amb)
     assert_bitwise(dts, f["dts"], "dt sequence")
     assert_bitwise(out, f["out"], name)
+
+
+# ---------------------------------------------------------------- post-step sources
+def test_sources_golden():
+    """Cooling/heating, ion chemistry, their chain with two-temperature, and the nozzle's chemistry
+    fill, bitwise against the reference's outputs."""
+    from tests import sources_cases as S
+    f = S.load()
+    for c, dt, cfg, pr, out in S.cooling_cases(f):
+        pr = pr.copy()
+        P = O.make_params("hydro", pr.shape[1:], nv=pr.shape[0], ng=cfg["NG"])
+        O.apply_cooling_heating(P, pr, dt, cfg)
+        assert_bitwise(pr[4], out, f"cooling {c}")
+    n_changed = 0
+    for c, dt, cfg, pr, out, nvb in S.chem_cases(f):
+        pr0 = pr.copy(); pr = pr.copy()
+        P = O.make_params("rmhd" if nvb == 9 else "hydro", pr.shape[1:], nv=pr.shape[0], ng=cfg["NG"])
+        O.apply_ion_chemistry(P, pr, dt, cfg)
+        assert_bitwise(pr[[4, nvb, nvb + 1, nvb + 2]], out, f"chemistry {c}")
+        n_changed += int((out[0] != pr0[4]).sum())
+    assert n_changed > 100  # the cases move p, not only the fractions
+    c, dt, cfg, pr, out, nvb = list(S.chem_cases(f))[-1]
+    assert (out[0] == 1e-20).sum() >= 2 and (out[1] == 1.0).sum() >= 10  # floors and clamps are covered
+    pr = f["chain_in"].copy()
+    cfg = dict(S.CHAIN_CFG, NG=int(f["ng"]))
+    P = O.make_params("hydro", pr.shape[1:], nv=pr.shape[0], ng=cfg["NG"])
+    O.apply_cooling_heating(P, pr, S.CHAIN_DT, cfg)
+    O.apply_two_temperature(P, pr, S.CHAIN_DT, cfg["TEI_TAU"], cfg["THERMO_OFFSET"])
+    O.apply_ion_chemistry(P, pr, S.CHAIN_DT, cfg)
+    assert_bitwise(pr[[4, 6, 7, 8, 9, 10]], f["chain_out"], "chain")
+    for c, name, cfg, pr, out, (NX, NY, NZ, ng) in S.nozzle_cases(f):
+        pr = pr.copy()
+        P = O.make_params(cfg["PHYSICS"], pr.shape[1:], nv=pr.shape[0], ng=ng)
+        draws = np.random.default_rng(1234).random(10000)
+        O.apply_nozzle_left_x(P, pr, 1.0 / NX, 1.0 / NY, 1.0 / NZ, NY, NZ, 0.45, 0.55, cfg, draws)
+        assert_bitwise(pr[:, :ng], out, f"nozzle {name}")
+
+
+def test_sources_oracle_properties():
+    """What the mathematics fixes, independent of the fixture: cooling is p + dt (H - L rho p)
+    clamped to [1e-12, P_MAX]; chemistry with zero rates (CHEM_RATE_SCALE = 0) only clamps the
+    fractions (xHeII <= 1 - xHeIII) and leaves p unchanged; an ionizing step conserves nothing
+    but moves p by -(gamma - 1) dE / T_unit, checked against a direct evaluation of the
+    energy term from the fractions the oracle returns."""
+    rng = np.random.default_rng(5)
+    ng = 2
+    shape = (7, 6, 5)
+    pr = np.zeros((8,) + shape)
+    pr[0] = rng.uniform(0.1, 2.0, shape)
+    pr[4] = rng.uniform(1e-3, 1.0, shape)
+    pr[5:8] = rng.uniform(-0.3, 1.3, (3,) + shape)
+    core = (slice(ng, -ng),) * 3
+    P = O.make_params("hydro", shape, nv=8, ng=ng)
+    q = pr.copy()
+    O.apply_cooling_heating(P, q, 0.1, dict(COOLING_ENABLED=True, COOLING_LAMBDA=3.0, HEATING_RATE=0.2,
+                                            P_MAX=0.9))
+    want = np.clip(pr[4] + 0.1 * (0.2 - 3.0 * pr[0] * pr[4]), 1e-12, 0.9)
+    assert np.allclose(q[4][core], want[core], rtol=1e-15, atol=0)
+    assert np.array_equal(q[4][:ng], pr[4][:ng])  # ghosts untouched
+    q = pr.copy()
+    cfg = dict(CHEMISTRY_ENABLED=True, N_CHEM=3, CHEM_OFFSET=5, CHEM_RATE_SCALE=0.0)
+    O.apply_ion_chemistry(P, q, 0.1, cfg)
+    x = np.clip(pr[5:8], 0.0, 1.0)
+    x[1] = np.minimum(x[1], 1.0 - x[2])
+    assert np.array_equal(q[4][core], pr[4][core])
+    assert np.array_equal(q[5:8][(slice(None),) + core], x[(slice(None),) + core])
+    q = pr.copy()
+    cfg = dict(CHEMISTRY_ENABLED=True, N_CHEM=3, CHEM_OFFSET=5, CHEM_TUNIT_K=1e6, CHEM_RATE_SCALE=1e9,
+               CHEM_ENERGY_SCALE=1e-9, GAMMA=1.4)
+    O.apply_ion_chemistry(P, q, 0.05, cfg)
+    nH, nHe = 0.76 * pr[0], 0.24 * pr[0] / 4.0
+    dE = 1e-9 * (nH * 157809.1 * (q[5] - x[0]) + nHe * 285335.4 * (q[6] - x[1]) + nHe * 631515.0 * (q[7] - x[2]))
+    want = np.maximum(pr[4] - 0.4 * dE / 1e6, 1e-20)
+    assert np.allclose(q[4][core], want[core], rtol=1e-12, atol=1e-300)
+    assert (q[5:8][(slice(None),) + core] >= 0).all() and (q[5:8][(slice(None),) + core] <= 1).all()
+    assert np.abs(q[5:8] - x)[(slice(None),) + core].max() > 1e-3  # the rates did act
+    # collisional-ionization equilibrium is a fixed point of the hydrogen IMEX step for any n_e and
+    # dt: x* = C_H / (C_H + A_H) with the published fits (Cen 1992 collisional ionization, case-B
+    # recombination 2.59e-13 (T / 1e4)^-0.7); a wrong sign, rate or operand moves it
+    T = np.maximum(pr[4] / pr[0] * 1e6, 1.0)
+    C = 5.85e-11 * np.sqrt(T) * np.exp(-157809.1 / T) / (1.0 + np.sqrt(T / 1e5))
+    A = 2.59e-13 * (T / 1e4) ** -0.7
+    q = pr.copy()
+    q[5] = C / (C + A)
+    O.apply_ion_chemistry(P, q, 0.05, cfg)
+    assert np.allclose(q[5][core], (C / (C + A))[core], rtol=1e-13, atol=1e-15)
