@@ -58,8 +58,14 @@ class Chem(ctypes.Structure):
                 ("energy_scale", _dbl), ("tmin_k", _dbl), ("gamma", _dbl)]
 
 
+class Diss(ctypes.Structure):
+    _fields_ = [("zeta", _dbl), ("eta", _dbl), ("kappa", _dbl), ("tau_bulk", _dbl), ("tau_shear", _dbl),
+                ("tau_heat", _dbl), ("cap_frac", _dbl), ("p_max", _dbl), ("advect", _i32)]
+
+
 class Sources(ctypes.Structure):
-    _fields_ = [("cooling", _i32), ("cool", _dbl), ("heat", _dbl), ("p_cap", _dbl),
+    _fields_ = [("dissipation", _i32), ("diss", Diss),
+                ("cooling", _i32), ("cool", _dbl), ("heat", _dbl), ("p_cap", _dbl),
                 ("chemistry", _i32), ("chem", Chem)]
 
 
@@ -104,6 +110,7 @@ SIGNATURES = {
     "afd_cooling_heating": [_VP, _dbl, _dbl, _dbl, _dbl],
     "afd_ion_chemistry": [_VP, _dbl, ctypes.POINTER(Chem)],
     "afd_post_step_sources": [_VP, _dbl, ctypes.POINTER(Sources)],
+    "afd_causal_dissipation": [_VP, _dbl, ctypes.POINTER(Diss)],
     "afd_count_nonfinite": [_VP, ctypes.POINTER(_i64)],
     "afd_p2c_batch": [_VP, _i64, _D, _D],
     "afd_c2p_batch": [_VP, _i64, _D, _D, ctypes.POINTER(_i32)],

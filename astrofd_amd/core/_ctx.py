@@ -38,7 +38,10 @@ def domain(physics, full_shape, dxyz=(1.0, 1.0, 1.0), rk_order=2, ng=None, gamma
         over = dict(nx=fx - 2 * ng, ny=fy - 2 * ng, nz=fz - 2 * ng, ng=ng, dx=float(dxyz[0]), dy=float(dxyz[1]),
                     dz=float(dxyz[2]), rk_order=int(rk_order), nv=nv)
         nb = 9 if physics == "rmhd" else 5
-        over["n_passive"] = nv - nb
+        po = int(s2.get("PASSIVE_OFFSET", nb)) if physics == "hydro" else nb
+        po = po if nv >= po else nb  # arrays without the dissipative block
+        over["passive_offset"] = po
+        over["n_passive"] = nv - po
         if gamma is not None:
             over["gamma"] = float(gamma)
         # NX..NZ only size the settings-derived dx; the overrides set the grid
@@ -48,17 +51,18 @@ def domain(physics, full_shape, dxyz=(1.0, 1.0, 1.0), rk_order=2, ng=None, gamma
     return d
 
 
-def source_domain(full_shape, ng, two_temp=None):
+def source_domain(full_shape, ng, two_temp=None, dxyz=(1.0, 1.0, 1.0)):
     """A context for the post-step sources on arrays of `full_shape`.  They read their constants
     from the cfg of each call (core/source_terms.py, core/chemistry.py), so no configure() is
     needed; `two_temp` = (THERMO_OFFSET, TEI_TAU) or None."""
     nv, fx, fy, fz = (int(v) for v in full_shape)
-    key = ("sources", nv, fx, fy, fz, int(ng), two_temp)
+    key = ("sources", nv, fx, fy, fz, int(ng), two_temp, tuple(float(d) for d in dxyz))
     d = _CACHE.get(key)
     if d is None:
         nx, ny, nz = max(fx - 2 * ng, 1), max(fy - 2 * ng, 1), max(fz - 2 * ng, 1)
         s = dict(PHYSICS="hydro", NX=nx, NY=ny, NZ=nz, NG=int(ng), Lx=1.0, Ly=1.0, Lz=1.0)
-        over = dict(nv=nv, n_passive=nv - 5)
+        over = dict(nv=nv, n_passive=nv - 5, passive_offset=5, dx=float(dxyz[0]), dy=float(dxyz[1]),
+                    dz=float(dxyz[2]))
         if two_temp is not None:
             over.update(two_temperature=1, thermo_offset=int(two_temp[0]), tei_tau=float(two_temp[1]))
         d = Domain(params_from_settings(s, **over))

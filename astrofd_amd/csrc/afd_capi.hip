@@ -32,6 +32,7 @@ struct afd_ctx {
     Grid g{};
     int nb = 5;
     bool carried = false, passive_flux = false;
+    int diss = 0;  // FaceArgs::diss of the SRHD faces
     size_t fsize = 8;
     hipStream_t stream = nullptr;
     double *P[3] = {nullptr, nullptr, nullptr};
@@ -172,13 +173,18 @@ extern "C" const char *afd_version(void) { return "astrofd_amd 0.1 (gfx950)"; }
 
 extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
     if (!params || !out) return fail(AFD_ERR_INVALID, "afd_create: null argument");
-    const afd_params &p = *params;
+    afd_params p = *params;
     if (p.physics != AFD_PHYS_HYDRO && p.physics != AFD_PHYS_RMHD) return fail(AFD_ERR_INVALID, "physics must be 0 or 1");
     int nb = p.physics == AFD_PHYS_RMHD ? 9 : 5;
     if (p.ng < 2) return fail(AFD_ERR_INVALID, "NG must be >= 2");
     if (p.nx < 2 || p.ny < 2 || p.nz < 2) return fail(AFD_ERR_INVALID, "grid too small");
-    if (p.n_passive < 0 || p.nv != nb + p.n_passive) return fail(AFD_ERR_INVALID, "nv must equal 5|9 + n_passive");
-    if (p.n_passive > 0 && p.passive_offset != nb) return fail(AFD_ERR_INVALID, "passive_offset must equal the base variable count");
+    // passive_offset < nb means "right after the base variables"; SRHD may hold fields between
+    // the base variables and the passives (the dissipative Pi, pi^ij, q^i at 5..14 with
+    // PASSIVE_OFFSET = 15, utils/settings.py:250-252), RMHD may not
+    if (p.passive_offset < nb) p.passive_offset = nb;
+    if (nb == 9 && p.passive_offset != 9) return fail(AFD_ERR_INVALID, "RMHD passives follow the 9 base variables");
+    if (p.n_passive < 0 || p.nv != p.passive_offset + p.n_passive)
+        return fail(AFD_ERR_INVALID, "nv must equal passive_offset + n_passive");
     if (p.nv > 32) return fail(AFD_ERR_INVALID, "nv > 32");
     if (p.recon_id < 0 || p.recon_id > 2) return fail(AFD_ERR_INVALID, "recon_id");
     if (p.limiter_id < 0 || p.limiter_id > 2) return fail(AFD_ERR_INVALID, "limiter_id");
@@ -223,6 +229,10 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
     c->carried = nb == 9 && p.recon_id != AFD_RECON_MUSCL;
     c->passive_flux = p.n_passive > 0 && (nb == 9 ? p.recon_id == AFD_RECON_MUSCL : p.recon_id != AFD_RECON_WENO);
     c->fsize = p.precision == AFD_PREC_MIXED ? 4 : 8;
+    if (nb == 5) {  // core/srhd_core.py:393-462: bulk pressure (MUSCL), dissipative fluxes (MUSCL 1x, PPM 2x)
+        int reps = p.nv >= 15 ? (p.recon_id == AFD_RECON_MUSCL ? 1 : (p.recon_id == AFD_RECON_PPM ? 2 : 0)) : 0;
+        c->diss = (p.recon_id == AFD_RECON_MUSCL && p.passive_offset > 5 ? 1 : 0) | (reps << 1);
+    }
     *out = nullptr;
     int rc;
 #define TRY(x)                       \
@@ -520,6 +530,7 @@ static FaceLaunch face_launch(afd_ctx *c, const double *P) {
     l.recon = c->p.recon_id;
     l.riemann = c->p.riemann_id;
     l.dir = 0;
+    l.diss = c->diss;
     l.mixed = c->p.precision == AFD_PREC_MIXED;
     return l;
 }
@@ -718,8 +729,40 @@ extern "C" int afd_ion_chemistry(afd_ctx *c, double dt, const afd_chem *chem) {
     return run_sources(c, a);
 }
 
+// core/dissipation.py:10-27: a no-op without terms, with a non-positive relaxation time, for
+// RMHD or with fewer than 15 variables.
+static bool diss_on(const afd_ctx *c, const afd_diss *d) {
+    if (d->zeta == 0.0 && d->eta == 0.0 && d->kappa == 0.0) return false;
+    if (std::min(std::min(d->tau_bulk, d->tau_shear), d->tau_heat) <= 0.0) return false;
+    return c->nb == 5 && c->p.nv >= 15;
+}
+
+static int run_diss(afd_ctx *c, double dt, const afd_diss *d) {
+    if (!diss_on(c, d)) return AFD_OK;
+    HIPCHK(hipSetDevice(c->p.device));
+    // the new fields are staged in variable planes 5..14 of a primitive buffer the step is not using
+    double *S = c->P[(c->cur + 1) % 3] + 5 * c->g.ncell;
+    DissArgs a{c->P[c->cur], S, c->g, c->p.ng, d->advect ? 1 : 0, dt, c->p.dx, c->p.dy, c->p.dz,
+               d->zeta, d->eta, d->kappa, d->tau_bulk, d->tau_shear, d->tau_heat, d->cap_frac * d->p_max};
+    dim3 grid((unsigned)((long)c->p.nx * c->p.ny), (unsigned)((c->p.nz + BLOCK - 1) / BLOCK));
+    Timed t(c, 4);
+    hipLaunchKernelGGL(k_diss, grid, dim3(BLOCK), 0, c->stream, a);
+    hipLaunchKernelGGL(k_diss_store, grid, dim3(BLOCK), 0, c->stream, a, c->P[c->cur]);
+    HIPCHK(hipGetLastError());
+    return AFD_OK;
+}
+
+extern "C" int afd_causal_dissipation(afd_ctx *c, double dt, const afd_diss *diss) {
+    if (!c || !diss) return fail(AFD_ERR_INVALID, "afd_causal_dissipation: null argument");
+    return run_diss(c, dt, diss);
+}
+
 extern "C" int afd_post_step_sources(afd_ctx *c, double dt, const afd_sources *src) {
     if (!c || !src) return fail(AFD_ERR_INVALID, "afd_post_step_sources: null argument");
+    if (src->dissipation) {  // solvers/srhd3d_mpi_muscl.py:323, before the pointwise sources
+        int rc = run_diss(c, dt, &src->diss);
+        if (rc) return rc;
+    }
     SourceArgs a{};
     a.dt = dt;
     if (src->cooling && (src->cool != 0.0 || src->heat != 0.0)) {

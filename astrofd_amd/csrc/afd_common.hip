@@ -245,6 +245,81 @@ __global__ __launch_bounds__(BLOCK) void k_sources(SourceArgs a) {
     }
 }
 
+__global__ __launch_bounds__(BLOCK) void k_diss(DissArgs a) {
+#pragma clang fp contract(off)
+    const Grid &g = a.g;
+    const int nyi = g.ny - 2 * a.ng, nzi = g.nz - 2 * a.ng;
+    const int kk = blockIdx.y * BLOCK + threadIdx.x;
+    if (kk >= nzi) return;
+    const int i = a.ng + (int)blockIdx.x / nyi, j = a.ng + (int)blockIdx.x % nyi, k = a.ng + kk;
+    const long c = cidx(g, i, j, k), n = g.ncell, sx = g.sx, sy = g.sy;
+    const double *P = a.P;
+    double f[10];
+#pragma unroll
+    for (int m = 0; m < 10; ++m) f[m] = P[(5 + m) * n + c];
+    const double vx = P[n + c], vy = P[2 * n + c], vz = P[3 * n + c];
+    if (a.advect) {  // core/dissipation.py:60-92, first-order upwind with the cell's velocity
+        const double hx = pymax(a.dx, 1e-12), hy = pymax(a.dy, 1e-12), hz = pymax(a.dz, 1e-12);
+        const long ox = vx > 0.0 ? -sx : sx, oy = vy > 0.0 ? -sy : sy, oz = vz > 0.0 ? -1 : 1;
+#pragma unroll
+        for (int m = 0; m < 10; ++m) {
+            const double *q = P + (5 + m) * n + c;
+            const double u0 = f[m];
+            double dudx = vx > 0.0 ? (u0 - q[ox]) / hx : (q[ox] - u0) / hx;
+            double dudy = vy > 0.0 ? (u0 - q[oy]) / hy : (q[oy] - u0) / hy;
+            double dudz = vz > 0.0 ? (u0 - q[oz]) / hz : (q[oz] - u0) / hz;
+            f[m] = u0 - a.dt * (vx * dudx + vy * dudy + vz * dudz);
+        }
+    }
+    const double i2x = 0.5 / pymax(a.dx, 1e-12), i2y = 0.5 / pymax(a.dy, 1e-12), i2z = 0.5 / pymax(a.dz, 1e-12);
+    const double *V1 = P + n + c, *V2 = P + 2 * n + c, *V3 = P + 3 * n + c;
+    const double dvx_dx = (V1[sx] - V1[-sx]) * i2x, dvx_dy = (V1[sy] - V1[-sy]) * i2y, dvx_dz = (V1[1] - V1[-1]) * i2z;
+    const double dvy_dx = (V2[sx] - V2[-sx]) * i2x, dvy_dy = (V2[sy] - V2[-sy]) * i2y, dvy_dz = (V2[1] - V2[-1]) * i2z;
+    const double dvz_dx = (V3[sx] - V3[-sx]) * i2x, dvz_dy = (V3[sy] - V3[-sy]) * i2y, dvz_dz = (V3[1] - V3[-1]) * i2z;
+    const double divv = dvx_dx + dvy_dy + dvz_dz;
+    const double cap = a.cap;
+    auto clamp = [cap](double x) { return x > cap ? cap : (x < -cap ? -cap : x); };
+    if (a.zeta != 0.0) {  // bulk, :122-130
+        double pi_eq = -a.zeta * divv;
+        f[0] = clamp((f[0] + a.dt * pi_eq / a.tau_bulk) / (1.0 + a.dt / a.tau_bulk));
+    }
+    if (a.eta != 0.0) {  // shear, :132-177 (pi^zz reset to -(pi^xx + pi^yy) before the caps)
+        const double third = 1.0 / 3.0, m2e = -2.0 * a.eta, ts = a.tau_shear;
+        const double e[6] = {m2e * (dvx_dx - third * divv), m2e * (dvy_dy - third * divv),
+                             m2e * (dvz_dz - third * divv), m2e * (0.5 * (dvx_dy + dvy_dx)),
+                             m2e * (0.5 * (dvx_dz + dvz_dx)), m2e * (0.5 * (dvy_dz + dvz_dy))};
+        double s6[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) s6[m] = (f[1 + m] + a.dt * e[m] / ts) / (1.0 + a.dt / ts);
+        s6[2] = -(s6[0] + s6[1]);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) f[1 + m] = clamp(s6[m]);
+    }
+    if (a.kappa != 0.0) {  // heat flux, :179-213, T = p / max(rho, 1e-12)
+        const double *R = P + c, *Pp = P + 4 * n + c;
+        auto T = [&](long o) { return Pp[o] / pymax(R[o], 1e-12); };
+        const double g3[3] = {(T(sx) - T(-sx)) * i2x, (T(sy) - T(-sy)) * i2y, (T(1) - T(-1)) * i2z};
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            double q_eq = -a.kappa * g3[m];
+            f[7 + m] = clamp((f[7 + m] + a.dt * q_eq / a.tau_heat) / (1.0 + a.dt / a.tau_heat));
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 10; ++m) a.S[m * n + c] = f[m];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_diss_store(DissArgs a, double *P) {
+    const Grid &g = a.g;
+    const int nyi = g.ny - 2 * a.ng, nzi = g.nz - 2 * a.ng;
+    const int kk = blockIdx.y * BLOCK + threadIdx.x;
+    if (kk >= nzi) return;
+    const int i = a.ng + (int)blockIdx.x / nyi, j = a.ng + (int)blockIdx.x % nyi, k = a.ng + kk;
+    const long c = cidx(g, i, j, k), n = g.ncell;
+#pragma unroll
+    for (int m = 0; m < 10; ++m) P[(5 + m) * n + c] = a.S[m * n + c];
+}
+
 __global__ __launch_bounds__(BLOCK) void k_max_speed(MaxSpeedArgs a) {
     const Grid &g = a.g;
     __shared__ double red[BLOCK];

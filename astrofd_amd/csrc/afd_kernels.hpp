@@ -53,6 +53,7 @@ template <typename T> struct FaceArgs {
     Phys<T> ph;
     double p_max, v_max;
     int lim;
+    int diss;          // SRHD dissipative hooks: bit 0 bulk pressure on the face pressures, bits 1-2 flux repeats
 };
 
 // ------------------------------------------------------------------ reconstruction
@@ -111,6 +112,32 @@ template <int DIR> AFD_D bool face_coords(const Grid &g, long t, int &i, int &j,
 }
 
 // ------------------------------------------------------------------ SRHD face
+// apply_diss_flux_x/y/z (core/srhd_core.py:401-462): Pi, pi^ij and q^i of `cell` (fields 5..14)
+// added to the physical flux F of a face state q (its velocities).
+template <int DIR, typename T>
+AFD_D void diss_flux(const double *__restrict__ P, long n, long cell, const T *q, T *F) {
+    const T pi = (T)P[5 * n + cell], pxx = (T)P[6 * n + cell], pyy = (T)P[7 * n + cell], pzz = (T)P[8 * n + cell];
+    const T pxy = (T)P[9 * n + cell], pxz = (T)P[10 * n + cell], pyz = (T)P[11 * n + cell];
+    const T qx = (T)P[12 * n + cell], qy = (T)P[13 * n + cell], qz = (T)P[14 * n + cell];
+    const T vx = q[1], vy = q[2], vz = q[3];
+    if (DIR == 0) {
+        F[1] += pi + pxx;
+        F[2] += pxy;
+        F[3] += pxz;
+        F[4] += vx * (pi + pxx) + vy * pxy + vz * pxz + qx;
+    } else if (DIR == 1) {
+        F[1] += pxy;
+        F[2] += pi + pyy;
+        F[3] += pyz;
+        F[4] += vx * pxy + vy * (pi + pyy) + vz * pyz + qy;
+    } else {
+        F[1] += pxz;
+        F[2] += pyz;
+        F[3] += pi + pzz;
+        F[4] += vx * pxz + vy * pyz + vz * (pi + pzz) + qz;
+    }
+}
+
 template <int RECON, int RIEM, int DIR, typename T>
 __global__ __launch_bounds__(BLOCK) void k_face_srhd(FaceArgs<T> a) {
     const Grid &g = a.g;
@@ -120,6 +147,10 @@ __global__ __launch_bounds__(BLOCK) void k_face_srhd(FaceArgs<T> a) {
     double qLd[5], qRd[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) recon_face<RECON, T>(a.P + v * g.ncell, c, s, a.lim, qLd[v], qRd[v]);
+    if (a.diss & 1) {  // add_pi_lr, core/srhd_core.py:394-399 (MUSCL faces, PASSIVE_OFFSET > 5)
+        qLd[4] += a.P[5 * g.ncell + c - s];
+        qRd[4] += a.P[5 * g.ncell + c];
+    }
     srhd_floor(a.p_max, a.v_max, qLd);
     srhd_floor(a.p_max, a.v_max, qRd);
     T WL = (T)lorentz_guarded<sizeof(T) == 4>(qLd[1] * qLd[1] + qLd[2] * qLd[2] + qLd[3] * qLd[3]);
@@ -134,6 +165,10 @@ __global__ __launch_bounds__(BLOCK) void k_face_srhd(FaceArgs<T> a) {
     srhd_p2c_W(a.ph, qR, WR, UR);
     srhd_flux<DIR>(qL, UL, FL);
     srhd_flux<DIR>(qR, UR, FR);
+    for (int r = 0; r < (a.diss >> 1); ++r) {  // apply_diss_flux: once per MUSCL face, twice per PPM face
+        diss_flux<DIR>(a.P, g.ncell, c - s, qL, FL);
+        diss_flux<DIR>(a.P, g.ncell, c, qR, FR);
+    }
     T csL = srhd_sound_speed(a.ph, qL[0], qL[4]), csR = srhd_sound_speed(a.ph, qR[0], qR[4]);
     T lmL, lpL, lmR, lpR;
     eig_speeds(qL[1 + DIR], csL, lmL, lpL);
@@ -526,6 +561,9 @@ __global__ __launch_bounds__(BLOCK) void k_update(UpdArgs<F> a) {
 #pragma unroll
             for (int v = 0; v < NB; ++v) a.Pout[v * g.ncell + c] = q[v];
         }
+        // SRHD fields between the base variables and the passives (the dissipative Pi, pi^ij,
+        // q^i) pass through the stage unchanged: core/srhd_core.py:1308-1309, 1326-1327
+        for (int v = NB; v < a.po; ++v) a.Pout[(long)v * g.ncell + c] = a.Pin[(long)v * g.ncell + c];
         for (int t = 0; t < a.np; ++t) {
             long o = (long)(a.po + t) * g.ncell + c;
             double rt = a.rhs_pass ? a.rhs_pass[t * g.ncell + c] : 0.0;
@@ -626,6 +664,20 @@ struct SourceArgs {
     double X, Y, tunit, rate_scale, energy_scale, tmin, gamma;
 };
 __global__ void k_sources(SourceArgs a);
+
+// core/dissipation.py:5-215: upwind advection of the ten dissipative fields (from the state as
+// it was, :47-103) and their IMEX relaxation (:105-213), interior cells; the new fields go to S
+// (10 planes of ncell) and k_diss_store copies them back, so that every cell reads old values.
+struct DissArgs {
+    const double *P;
+    double *S;
+    Grid g;
+    int ng, advect;
+    double dt, dx, dy, dz;
+    double zeta, eta, kappa, tau_bulk, tau_shear, tau_heat, cap;
+};
+__global__ void k_diss(DissArgs a);
+__global__ void k_diss_store(DissArgs a, double *P);
 
 // ------------------------------------------------------------------ reductions
 // core/srhd_core.py:1267-1281 over cells [1, n-1)^3; one partial maximum per block

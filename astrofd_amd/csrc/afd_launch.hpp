@@ -15,7 +15,7 @@ struct FaceLaunch {
     Phys<double> phd;
     Phys<float> phf;
     double p_max, v_max;
-    int lim, recon, riemann, dir;
+    int lim, recon, riemann, dir, diss;
     bool mixed;
 };
 
@@ -30,6 +30,7 @@ template <typename T> inline FaceArgs<T> face_args(const FaceLaunch &l) {
     a.p_max = l.p_max;
     a.v_max = l.v_max;
     a.lim = l.lim;
+    a.diss = l.diss;
     if constexpr (sizeof(T) == 8) a.ph = l.phd;
     else a.ph = l.phf;
     return a;

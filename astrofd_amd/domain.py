@@ -43,16 +43,14 @@ def params_from_settings(s, device=0, **override):
     phys = str(s.get("PHYSICS", "hydro")).lower()
     if phys not in ("hydro", "rmhd"):
         raise NotImplementedError(f"PHYSICS={phys!r}: only the SRHD (hydro) and RMHD step are on this path")
-    if s.get("DISSIPATION_ENABLED", False):
-        raise NotImplementedError("DISSIPATION_ENABLED: the dissipative operator is not on this path")
     rm = phys == "rmhd"
     nb = 9 if rm else 5
     rc = recon_id(s.get("RECON", "muscl"))
     p = L.Params()
     p.physics = L.AFD_PHYS_RMHD if rm else L.AFD_PHYS_HYDRO
     p.n_passive = int(s.get("N_PASSIVE", 0))
-    p.passive_offset = int(s.get("PASSIVE_OFFSET", nb))
-    p.nv = nb + p.n_passive
+    p.passive_offset = int(s.get("PASSIVE_OFFSET", nb))  # 15 for SRHD with dissipation (fields 5..14)
+    p.nv = p.passive_offset + p.n_passive
     p.nx, p.ny, p.nz, p.ng = int(s["NX"]), int(s["NY"]), int(s["NZ"]), int(s["NG"])
     p.dx = s["Lx"] / s["NX"]
     p.dy = s["Ly"] / s["NY"]
@@ -137,10 +135,27 @@ def chem_from_settings(s):
                   gamma=float(s.get("GAMMA", 5.0 / 3.0)))
 
 
+def diss_from_settings(s):
+    """The constants core/dissipation.py:16-45 reads (a missing, None or zero RELAX_TAU_* falls
+    back to RELAX_TAU, :19-21)."""
+    base = s.get("RELAX_TAU", 0.1)
+
+    def tau(key):
+        return float(s.get(key, base) or base)
+    return L.Diss(zeta=float(s.get("BULK_ZETA", 0.0)), eta=float(s.get("SHEAR_ETA", 0.0)),
+                  kappa=float(s.get("HEAT_KAPPA", 0.0)), tau_bulk=tau("RELAX_TAU_BULK"),
+                  tau_shear=tau("RELAX_TAU_SHEAR"), tau_heat=tau("RELAX_TAU_HEAT"),
+                  cap_frac=float(s.get("DISSIPATION_CAP_FRAC", 0.5)), p_max=float(s.get("P_MAX", 1.0)),
+                  advect=int(bool(s.get("DISSIPATION_ADVECT", True))))
+
+
 def sources_from_settings(s):
     """The post-step sources the driver applies (solvers/srhd3d_mpi_muscl.py:324-326) and their
-    settings gates: core/source_terms.py:6-11, core/chemistry.py:48-53."""
+    settings gates: core/dissipation.py:10-13, core/source_terms.py:6-11, core/chemistry.py:48-53."""
     src = L.Sources()
+    src.dissipation = int(bool(s.get("DISSIPATION_ENABLED", False)) and
+                          str(s.get("PHYSICS", "hydro")).lower() in ("hydro", "grhd"))
+    src.diss = diss_from_settings(s)
     src.cooling = int(bool(s.get("COOLING_ENABLED", False)))
     src.cool = float(s.get("COOLING_LAMBDA", 0.0))
     src.heat = float(s.get("HEATING_RATE", 0.0))
@@ -261,8 +276,15 @@ class Domain:
             chem = chem_from_settings(chem)
         L.check(self.lib.afd_ion_chemistry(self.h, float(dt), ctypes.byref(chem)), "afd_ion_chemistry")
 
+    def causal_dissipation(self, dt, diss):
+        """`diss`: an L.Diss, or a settings dict (diss_from_settings)."""
+        if isinstance(diss, dict):
+            diss = diss_from_settings(diss)
+        L.check(self.lib.afd_causal_dissipation(self.h, float(dt), ctypes.byref(diss)), "afd_causal_dissipation")
+
     def post_step_sources(self, dt, sources=None):
-        """Cooling/heating, two-temperature and chemistry in one pass (the driver's order)."""
+        """Dissipation, then cooling/heating, two-temperature and chemistry in one pass (the
+        driver's order)."""
         src = self.sources if sources is None else sources
         L.check(self.lib.afd_post_step_sources(self.h, float(dt), ctypes.byref(src)), "afd_post_step_sources")
 

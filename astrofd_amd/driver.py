@@ -4,8 +4,8 @@
 
 Same settings keys and CLI (utils/settings.py:100-107, plus --max-steps), the same loop
 (solvers/srhd3d_mpi_muscl.py:276-330): CFL time step (SRHD: max characteristic speed; RMHD:
-amax = 1), ghost/nozzle fills, one SSPRK2/3 step, fills again, post-step sources (cooling/heating,
-two-temperature, ion chemistry, fused into one pass over the interior), progress
+amax = 1), ghost/nozzle fills, one SSPRK2/3 step, fills again, post-step operators (Israel-Stewart
+dissipation; cooling/heating, two-temperature and ion chemistry fused into one pass), progress
 print, NaN check, npz outputs and checkpoints with the reference's keys and meta, and the
 diagnostics CSVs of utils/diagnostics.py (max Gamma and inlet flux, centerline, div B), whose
 reductions run on the GPU.  The whole
@@ -92,6 +92,9 @@ def output_fields(s, pr):
     base = dict(rho=pr[0], vx=pr[1], vy=pr[2], vz=pr[3], p=pr[4])
     if s["PHYSICS"] == "rmhd":
         base.update(Bx=pr[5], By=pr[6], Bz=pr[7], psi=pr[8])
+    elif int(s.get("TRACER_OFFSET", 5)) > 5:  # the dissipative fields (:362-372)
+        for m, name in enumerate(("pi", "pixx", "piyy", "pizz", "pixy", "pixz", "piyz", "qx", "qy", "qz")):
+            base[name] = pr[5 + m]
     base.update(f)
     return base
 
@@ -179,7 +182,7 @@ def run(s, log=print):
             d.apply_bcs(rng)
             d.step(dt)
             d.apply_bcs(rng)
-            d.post_step_sources(dt)  # cooling/heating, two-temperature, chemistry (:324-326)
+            d.post_step_sources(dt)  # dissipation, cooling/heating, two-temperature, chemistry (:323-326)
             t += dt
             step += 1
             if step % int(s["PRINT_EVERY"]) == 0 or abs(t - T_END) < 1e-14:

@@ -36,7 +36,7 @@ def apply_derived_ic_keys(s):
 def init_block(s):
     """solvers/srhd3d_mpi_muscl.py:128-163 for one rank holding the whole domain."""
     rm = s["PHYSICS"] in ("rmhd", "grmhd")
-    nv = 9 if rm else 5
+    nv = 9 if rm else (15 if s.get("DISSIPATION_ENABLED", False) else 5)  # Pi, pi^ij, q^i at 5..14, zero
     nv += int(s.get("N_TRACERS", 0)) + int(s.get("N_THERMO", 0)) + int(s.get("N_CHEM", 0))
     ng = int(s["NG"])
     pr = np.zeros((nv, s["NX"] + 2 * ng, s["NY"] + 2 * ng, s["NZ"] + 2 * ng), dtype=np.float64)

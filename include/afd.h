@@ -59,7 +59,8 @@ typedef struct afd_params {
     double p_max, v_max;
     double glm_ch, glm_cp;
     int32_t limiter_id, recon_id, riemann_id, rk_order;
-    int32_t n_passive, passive_offset;
+    int32_t n_passive, passive_offset; /* nv = passive_offset + n_passive; SRHD fields in [5, passive_offset)
+                                          (the dissipative 5..14) pass through the step unchanged */
     int32_t n_resist;         /* applications of the resistive Laplacian per RHS (reference: 1 MUSCL, 2 PPM, 0 WENO) */
     double resistivity;
     int32_t precision;        /* AFD_PREC_* */
@@ -174,15 +175,33 @@ typedef struct afd_chem {
  * the caller checks CHEMISTRY_ENABLED and N_CHEM >= 3 (core/chemistry.py:48-53). */
 int afd_ion_chemistry(afd_ctx *ctx, double dt, const afd_chem *chem);
 
+/* Constants of core/dissipation.py:16-45, read from the settings by the caller (a missing, None
+ * or zero RELAX_TAU_* falls back to RELAX_TAU, :19-21). */
+typedef struct afd_diss {
+    double zeta, eta, kappa;                /* BULK_ZETA, SHEAR_ETA, HEAT_KAPPA */
+    double tau_bulk, tau_shear, tau_heat;   /* relaxation times */
+    double cap_frac, p_max;                 /* |Pi|, |pi^ij|, |q^i| <= DISSIPATION_CAP_FRAC * P_MAX */
+    int32_t advect;                         /* DISSIPATION_ADVECT */
+} afd_diss;
+
+/* core/dissipation.py:5-215 (apply_causal_dissipation) on the current state's interior: upwind
+ * advection of the ten dissipative fields 5..14 (if advect) and their IMEX relaxation towards the
+ * Navier-Stokes targets.  A no-op for RMHD, nv < 15, no terms or a non-positive relaxation time;
+ * the caller checks DISSIPATION_ENABLED.  The grid spacing is the context's. */
+int afd_causal_dissipation(afd_ctx *ctx, double dt, const afd_diss *diss);
+
 typedef struct afd_sources {
+    int32_t dissipation;      /* DISSIPATION_ENABLED (hydro) */
+    afd_diss diss;
     int32_t cooling;          /* COOLING_ENABLED */
     double cool, heat, p_cap; /* as afd_cooling_heating */
     int32_t chemistry;        /* CHEMISTRY_ENABLED and N_CHEM >= 3 */
     afd_chem chem;
 } afd_sources;
 
-/* The driver's post-step sources, solvers/srhd3d_mpi_muscl.py:324-326 (apply_cooling_heating,
- * apply_two_temperature, apply_ion_chemistry), in one pass over the interior that applies them
+/* The driver's post-step operators, solvers/srhd3d_mpi_muscl.py:323-326: causal dissipation (if
+ * sources->dissipation; a stencil, so two passes of its own), then apply_cooling_heating,
+ * apply_two_temperature and apply_ion_chemistry in one pass over the interior that applies them
  * to each cell in that order.  Two-temperature follows the context's parameters. */
 int afd_post_step_sources(afd_ctx *ctx, double dt, const afd_sources *sources);
 

@@ -51,6 +51,11 @@ class Nozzle(ctypes.Structure):
     ]
 
 
+class Diss(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("zeta", "eta", "kappa", "tau_bulk", "tau_shear", "tau_heat",
+                                               "cap_frac", "p_max")] + [("advect", ctypes.c_int)]
+
+
 class Chem(ctypes.Structure):
     _fields_ = [("offset", ctypes.c_int)] + [(n, ctypes.c_double) for n in
                                             ("X", "Y", "tunit", "rate_scale", "energy_scale", "tmin", "gamma")]
@@ -88,6 +93,7 @@ def lib():
             "orc_apply_two_temperature": [P, _D, ctypes.c_double, ctypes.c_double, ctypes.c_int],
             "orc_apply_cooling_heating": [P, _D] + [ctypes.c_double] * 4,
             "orc_apply_ion_chemistry": [P, _D, ctypes.c_double, ctypes.POINTER(Chem)],
+            "orc_apply_causal_dissipation": [P, _D] + [ctypes.c_double] * 4 + [ctypes.POINTER(Diss)],
         }
         for name, args in sig.items():
             getattr(L, name).argtypes = args
@@ -331,6 +337,29 @@ def apply_ion_chemistry(P, pr, dt, cfg):
     Q = with_shape(P, pr)
     c = chem_from_cfg(cfg)
     lib().orc_apply_ion_chemistry(ctypes.byref(Q), _p(pr), dt, ctypes.byref(c))
+
+
+def diss_from_cfg(cfg):
+    """The constants core/dissipation.py:16-45 reads (a missing, None or zero RELAX_TAU_* falls
+    back to RELAX_TAU, :19-21)."""
+    base = cfg.get("RELAX_TAU", 0.1)
+
+    def tau(key):
+        return float(cfg.get(key, base) or base)
+    return Diss(zeta=float(cfg.get("BULK_ZETA", 0.0)), eta=float(cfg.get("SHEAR_ETA", 0.0)),
+                kappa=float(cfg.get("HEAT_KAPPA", 0.0)), tau_bulk=tau("RELAX_TAU_BULK"),
+                tau_shear=tau("RELAX_TAU_SHEAR"), tau_heat=tau("RELAX_TAU_HEAT"),
+                cap_frac=float(cfg.get("DISSIPATION_CAP_FRAC", 0.5)), p_max=float(cfg.get("P_MAX", 1.0)),
+                advect=int(bool(cfg.get("DISSIPATION_ADVECT", True))))
+
+
+def apply_causal_dissipation(P, pr, dt, dx, dy, dz, cfg):
+    """core/dissipation.py:5-215 on `pr` in place (the cfg gates included)."""
+    if not cfg.get("DISSIPATION_ENABLED", False) or cfg.get("PHYSICS") not in ("hydro", "grhd"):
+        return
+    Q = with_shape(P, pr)
+    d = diss_from_cfg(cfg)
+    lib().orc_apply_causal_dissipation(ctypes.byref(Q), _p(pr), dt, dx, dy, dz, ctypes.byref(d))
 
 
 def num_threads():

@@ -322,11 +322,47 @@ static void srhd_floor(double p_max, double v_max, double *q) {
     }
 }
 
+/* apply_diss_flux_x/y/z (core/srhd_core.py:401-462): the bulk pressure Pi, the shear stress
+ * pi^ij and the heat flux q^i of cell `cell` (fields 5..14) added to a physical flux, with the
+ * face state's velocities. */
+static void diss_flux(const orc_params *P, const double *pr, size_t cell, int dir, const double *q, double *F) {
+    const size_t N = (size_t)NXF * NYF * NZF;
+    const double pi = pr[5 * N + cell], pixx = pr[6 * N + cell], piyy = pr[7 * N + cell], pizz = pr[8 * N + cell];
+    const double pixy = pr[9 * N + cell], pixz = pr[10 * N + cell], piyz = pr[11 * N + cell];
+    const double qx = pr[12 * N + cell], qy = pr[13 * N + cell], qz = pr[14 * N + cell];
+    const double vx = q[1], vy = q[2], vz = q[3];
+    if (dir == 0) {
+        F[1] += pi + pixx;
+        F[2] += pixy;
+        F[3] += pixz;
+        F[4] += vx * (pi + pixx) + vy * pixy + vz * pixz + qx;
+    } else if (dir == 1) {
+        F[1] += pixy;
+        F[2] += pi + piyy;
+        F[3] += piyz;
+        F[4] += vx * pixy + vy * (pi + piyy) + vz * piyz + qy;
+    } else {
+        F[1] += pixz;
+        F[2] += piyz;
+        F[3] += pi + pizz;
+        F[4] += vx * pixz + vy * piyz + vz * (pi + pizz) + qz;
+    }
+}
+
 /* One SRHD face: floors, p2c, physical fluxes, eigen-speed bounds and the Riemann solver
  * (core/srhd_core.py:536-549 and the identical blocks of every reconstruction).
- * qL/qR are floored in place; returns sL/sR. */
-static void srhd_face(const orc_params *P, int dir, double *qL, double *qR, double *F, double *sLo, double *sRo) {
+ * qL/qR are floored in place; returns sL/sR.  With `pr` (the RHS input) the dissipative hooks
+ * of the face's cells cL | cR (flat indices of variable 0) apply: add_pi_lr (:394-399, MUSCL
+ * faces only, when PASSIVE_OFFSET > 5) before the floors, and apply_diss_flux (nv >= 15) once
+ * per MUSCL face and twice per PPM face (:958-961); the WENO RHS has neither. */
+static void srhd_face(const orc_params *P, int dir, double *qL, double *qR, double *F, double *sLo, double *sRo,
+                      const double *pr, size_t cL, size_t cR) {
     double g = P->gamma;
+    const size_t N = (size_t)NXF * NYF * NZF;
+    if (pr && P->recon_id == 0 && P->passive_offset > 5) {
+        qL[4] += pr[5 * N + cL];
+        qR[4] += pr[5 * N + cR];
+    }
     srhd_floor(P->p_max, P->v_max, qL);
     srhd_floor(P->p_max, P->v_max, qR);
     double UL[5], UR[5], FL[5], FR[5];
@@ -334,6 +370,13 @@ static void srhd_face(const orc_params *P, int dir, double *qL, double *qR, doub
     srhd_p2c(g, qR, UR);
     srhd_flux(g, dir, qL, FL);
     srhd_flux(g, dir, qR, FR);
+    if (pr && P->nv >= 15) {
+        const int reps = P->recon_id == 0 ? 1 : (P->recon_id == 1 ? 2 : 0);
+        for (int r = 0; r < reps; ++r) {
+            diss_flux(P, pr, cL, dir, qL, FL);
+            diss_flux(P, pr, cR, dir, qR, FR);
+        }
+    }
     double csL = srhd_sound_speed(g, qL[0], qL[4]), csR = srhd_sound_speed(g, qR[0], qR[4]);
     double lmL, lpL, lmR, lpR;
     eig_speeds(qL[1 + dir], csL, &lmL, &lpL);
@@ -364,7 +407,7 @@ void orc_srhd_riemann(const orc_params *P, int dir, const double *primL, const d
     double qL[5], qR[5], sL, sR;
     memcpy(qL, primL, sizeof qL);
     memcpy(qR, primR, sizeof qR);
-    srhd_face(P, dir, qL, qR, F, &sL, &sR);
+    srhd_face(P, dir, qL, qR, F, &sL, &sR, NULL, 0, 0);
 }
 
 /* ---------------------------------------------------------------------
@@ -470,8 +513,9 @@ static void srhd_rhs_muscl(const orc_params *P, const double *pr, double *rhs, d
                 for (int k = k0; k < k1; ++k) {
                     int ip = i + (dir == 0), jp = j + (dir == 1), kp = k + (dir == 2);
                     double qL[5], qR[5], FA[5], FB[5], sL, sR, sL2, sR2;
+                    const size_t sd = dstride(P, dir), ca = IX(0, i, j, k), cb = IX(0, ip, jp, kp);
                     face_states(P, pr, 5, dir, i, j, k, qL, qR);
-                    srhd_face(P, dir, qL, qR, FA, &sL, &sR);
+                    srhd_face(P, dir, qL, qR, FA, &sL, &sR, pr, ca - sd, ca);
                     double vLa = qL[1 + dir], vRa = qR[1 + dir];
                     double trL[8];
                     /* x: WENO, PPM and MUSCL blocks each overwrite FxL_tr (:550-580); z: MUSCL only (:847-859) */
@@ -485,7 +529,7 @@ static void srhd_rhs_muscl(const orc_params *P, const double *pr, double *rhs, d
                         }
                     }
                     face_states(P, pr, 5, dir, ip, jp, kp, qL, qR);
-                    srhd_face(P, dir, qL, qR, FB, &sL2, &sR2);
+                    srhd_face(P, dir, qL, qR, FB, &sL2, &sR2, pr, cb - sd, cb);
                     double vLb = qL[1 + dir], vRb = qR[1 + dir];
                     if (np > 0 && dir != 1) {
                         for (int t = 0; t < np; ++t) {
@@ -518,8 +562,9 @@ static void srhd_rhs_ppm_weno(const orc_params *P, const double *pr, double *rhs
                 for (int dir = 0; dir < 3; ++dir) {
                     int ip = i + (dir == 0), jp = j + (dir == 1), kp = k + (dir == 2);
                     double qL[5], qR[5], FA[5], FB[5], sL, sR, sL2, sR2, trL[8];
+                    const size_t sd = dstride(P, dir), ca = IX(0, i, j, k), cb = IX(0, ip, jp, kp);
                     face_states(P, pr, 5, dir, i, j, k, qL, qR);
-                    srhd_face(P, dir, qL, qR, FA, &sL, &sR);
+                    srhd_face(P, dir, qL, qR, FA, &sL, &sR, pr, ca - sd, ca);
                     double vLa = qL[1 + dir], vRa = qR[1 + dir];
                     /* PPM passives: y has WENO, PPM and MUSCL blocks (:1007-1037), z has WENO and PPM (:1110-1127) */
                     int pas = ppm && np > 0 && dir != 0;
@@ -531,7 +576,7 @@ static void srhd_rhs_ppm_weno(const orc_params *P, const double *pr, double *rhs
                         }
                     }
                     face_states(P, pr, 5, dir, ip, jp, kp, qL, qR);
-                    srhd_face(P, dir, qL, qR, FB, &sL2, &sR2);
+                    srhd_face(P, dir, qL, qR, FB, &sL2, &sR2, pr, cb - sd, cb);
                     double vLb = qL[1 + dir], vRb = qR[1 + dir];
                     if (pas) {
                         for (int t = 0; t < np; ++t) {
@@ -1371,4 +1416,94 @@ void orc_apply_ion_chemistry(const orc_params *P, double *pr, double dt, const o
                 PR(off + 1, i, j, k) = xHeII_new;
                 PR(off + 2, i, j, k) = xHeIII_new;
             }
+}
+
+/* core/dissipation.py:5-215 (apply_causal_dissipation), hydro only, nv >= 15: first-order upwind
+ * advection of the ten dissipative fields with the cell velocity (:47-103, from a copy of the
+ * fields), then the IMEX relaxation of Pi, pi^ij and q^i towards their Navier-Stokes targets
+ * from centred velocity and temperature (p / rho) gradients, each clamped to +-cap_frac P_MAX;
+ * pi^zz is reset to -(pi^xx + pi^yy).  Interior cells only. */
+void orc_apply_causal_dissipation(const orc_params *P, double *pr, double dt, double dx, double dy, double dz,
+                                  const orc_diss *D) {
+    if ((D->zeta == 0.0 && D->eta == 0.0 && D->kappa == 0.0) ||
+        pymin(pymin(D->tau_bulk, D->tau_shear), D->tau_heat) <= 0.0)
+        return;
+    if (P->nv < 15) return;
+    const int ng = P->ng;
+    const size_t N = (size_t)NXF * NYF * NZF;
+    const double inv2dx = 0.5 / pymax(dx, 1e-12), inv2dy = 0.5 / pymax(dy, 1e-12), inv2dz = 0.5 / pymax(dz, 1e-12);
+    if (D->advect) {
+        double *adv = (double *)malloc(sizeof(double) * 10 * N);
+        memcpy(adv, pr + 5 * N, sizeof(double) * 10 * N);
+#pragma omp parallel for schedule(static)
+        for (int i = ng; i < NXF - ng; ++i)
+            for (int j = ng; j < NYF - ng; ++j)
+                for (int k = ng; k < NZF - ng; ++k) {
+                    double vx = PR(1, i, j, k), vy = PR(2, i, j, k), vz = PR(3, i, j, k);
+                    for (int f = 5; f < 15; ++f) {
+                        double u0 = PR(f, i, j, k), dudx, dudy, dudz;
+                        if (vx > 0.0) dudx = (u0 - PR(f, i - 1, j, k)) / pymax(dx, 1e-12);
+                        else dudx = (PR(f, i + 1, j, k) - u0) / pymax(dx, 1e-12);
+                        if (vy > 0.0) dudy = (u0 - PR(f, i, j - 1, k)) / pymax(dy, 1e-12);
+                        else dudy = (PR(f, i, j + 1, k) - u0) / pymax(dy, 1e-12);
+                        if (vz > 0.0) dudz = (u0 - PR(f, i, j, k - 1)) / pymax(dz, 1e-12);
+                        else dudz = (PR(f, i, j, k + 1) - u0) / pymax(dz, 1e-12);
+                        adv[(size_t)(f - 5) * N + IX(0, i, j, k)] = u0 - dt * (vx * dudx + vy * dudy + vz * dudz);
+                    }
+                }
+        memcpy(pr + 5 * N, adv, sizeof(double) * 10 * N);
+        free(adv);
+    }
+    const double cap = D->cap_frac * D->p_max;
+#define CLAMP(x) do { if ((x) > cap) (x) = cap; if ((x) < -cap) (x) = -cap; } while (0)
+#define TEMP(ii, jj, kk) (PR(4, ii, jj, kk) / pymax(PR(0, ii, jj, kk), 1e-12))
+#pragma omp parallel for schedule(static)
+    for (int i = ng; i < NXF - ng; ++i)
+        for (int j = ng; j < NYF - ng; ++j)
+            for (int k = ng; k < NZF - ng; ++k) {
+                double dvx_dx = (PR(1, i + 1, j, k) - PR(1, i - 1, j, k)) * inv2dx;
+                double dvx_dy = (PR(1, i, j + 1, k) - PR(1, i, j - 1, k)) * inv2dy;
+                double dvx_dz = (PR(1, i, j, k + 1) - PR(1, i, j, k - 1)) * inv2dz;
+                double dvy_dx = (PR(2, i + 1, j, k) - PR(2, i - 1, j, k)) * inv2dx;
+                double dvy_dy = (PR(2, i, j + 1, k) - PR(2, i, j - 1, k)) * inv2dy;
+                double dvy_dz = (PR(2, i, j, k + 1) - PR(2, i, j, k - 1)) * inv2dz;
+                double dvz_dx = (PR(3, i + 1, j, k) - PR(3, i - 1, j, k)) * inv2dx;
+                double dvz_dy = (PR(3, i, j + 1, k) - PR(3, i, j - 1, k)) * inv2dy;
+                double dvz_dz = (PR(3, i, j, k + 1) - PR(3, i, j, k - 1)) * inv2dz;
+                double divv = dvx_dx + dvy_dy + dvz_dz;
+                if (D->zeta != 0.0) {
+                    double pi_eq = -D->zeta * divv;
+                    double pi_new = (PR(5, i, j, k) + dt * pi_eq / D->tau_bulk) / (1.0 + dt / D->tau_bulk);
+                    CLAMP(pi_new);
+                    PR(5, i, j, k) = pi_new;
+                }
+                if (D->eta != 0.0) {
+                    const double third = 1.0 / 3.0, ts = D->tau_shear;
+                    double sxx = dvx_dx - third * divv, syy = dvy_dy - third * divv, szz = dvz_dz - third * divv;
+                    double sxy = 0.5 * (dvx_dy + dvy_dx), sxz = 0.5 * (dvx_dz + dvz_dx), syz = 0.5 * (dvy_dz + dvz_dy);
+                    double m2e = -2.0 * D->eta;
+                    double e[6] = {m2e * sxx, m2e * syy, m2e * szz, m2e * sxy, m2e * sxz, m2e * syz};
+                    double v[6];
+                    for (int m = 0; m < 6; ++m) v[m] = (PR(6 + m, i, j, k) + dt * e[m] / ts) / (1.0 + dt / ts);
+                    v[2] = -(v[0] + v[1]);
+                    for (int m = 0; m < 6; ++m) {
+                        CLAMP(v[m]);
+                        PR(6 + m, i, j, k) = v[m];
+                    }
+                }
+                if (D->kappa != 0.0) {
+                    double dTdx = (TEMP(i + 1, j, k) - TEMP(i - 1, j, k)) * inv2dx;
+                    double dTdy = (TEMP(i, j + 1, k) - TEMP(i, j - 1, k)) * inv2dy;
+                    double dTdz = (TEMP(i, j, k + 1) - TEMP(i, j, k - 1)) * inv2dz;
+                    double g[3] = {dTdx, dTdy, dTdz};
+                    for (int m = 0; m < 3; ++m) {
+                        double q_eq = -D->kappa * g[m];
+                        double qn = (PR(12 + m, i, j, k) + dt * q_eq / D->tau_heat) / (1.0 + dt / D->tau_heat);
+                        CLAMP(qn);
+                        PR(12 + m, i, j, k) = qn;
+                    }
+                }
+            }
+#undef CLAMP
+#undef TEMP
 }

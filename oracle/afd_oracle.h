@@ -103,6 +103,14 @@ typedef struct orc_chem {
     double X, Y, tunit, rate_scale, energy_scale, tmin, gamma;
 } orc_chem;
 void orc_apply_ion_chemistry(const orc_params *P, double *pr, double dt, const orc_chem *chem);
+typedef struct orc_diss {
+    double zeta, eta, kappa;                /* BULK_ZETA, SHEAR_ETA, HEAT_KAPPA */
+    double tau_bulk, tau_shear, tau_heat;   /* RELAX_TAU_* (resolved against RELAX_TAU) */
+    double cap_frac, p_max;                 /* DISSIPATION_CAP_FRAC, P_MAX */
+    int advect;                             /* DISSIPATION_ADVECT */
+} orc_diss;
+void orc_apply_causal_dissipation(const orc_params *P, double *pr, double dt, double dx, double dy, double dz,
+                                  const orc_diss *diss);
 
 /* number of OpenMP threads the whole-array operators use (1 when built without OpenMP) */
 int orc_num_threads(void);

@@ -48,7 +48,7 @@ def _install_numba_stub():
 
 _install_numba_stub()
 sys.path.insert(0, REF)
-from core import srhd_core, rmhd_core, boundary, nozzle, source_terms, chemistry  # noqa: E402
+from core import srhd_core, rmhd_core, boundary, nozzle, source_terms, chemistry, dissipation  # noqa: E402
 
 
 def _save(name, **arrays):
@@ -478,6 +478,61 @@ def gen_sources():
 
 
 # ---------------------------------------------------------------------------
+# Israel-Stewart dissipation: the SRHD RHS hooks (core/srhd_core.py:393-462), the
+# step with fields 5..14 carried (PASSIVE_OFFSET = 15) and the post-step operator
+# (core/dissipation.py:5-215)
+# ---------------------------------------------------------------------------
+DISS_RHS_CASES = [("muscl", "mc", "hllc"), ("muscl", "minmod", "hlle"), ("ppm", "mc", "hlle"), ("weno", "mc", "hllc")]
+DISS_STEP_CASES = [("muscl", "vanleer", "hllc", 3), ("ppm", "mc", "hlle", 2)]
+DISS_OP_CASES = [
+    (True, dict(BULK_ZETA=0.05, SHEAR_ETA=0.02, HEAT_KAPPA=0.01, RELAX_TAU=0.1)),
+    (True, dict(BULK_ZETA=0.05, RELAX_TAU_BULK=0.05, DISSIPATION_ADVECT=False)),
+    (True, dict(SHEAR_ETA=5.0, HEAT_KAPPA=2.0, RELAX_TAU=0.01, DISSIPATION_CAP_FRAC=0.002, P_MAX=2.0)),
+    (True, dict(BULK_ZETA=0.0, SHEAR_ETA=0.0, HEAT_KAPPA=0.0)),
+    (True, dict(BULK_ZETA=0.1, HEAT_KAPPA=0.3, RELAX_TAU_SHEAR=0.0, RELAX_TAU=0.2, RELAX_TAU_HEAT=None)),
+    (False, dict(BULK_ZETA=0.1, SHEAR_ETA=0.1, HEAT_KAPPA=0.1)),
+]
+
+
+def diss_state(shape, seed, npass=2):
+    """srhd_state with the ten dissipative fields (Pi, pi^ij, q^i ~ +-4e-3) at 5..14 and the
+    passives after them (PASSIVE_OFFSET = 15)."""
+    base = srhd_state(shape, seed, npass=npass)
+    diss = np.random.default_rng(seed + 1).uniform(-0.004, 0.004, (10,) + shape)
+    return np.concatenate([base[:5], diss, base[5:]])
+
+
+def gen_dissipation():
+    # one shared input per group keeps the fixture small: 10^3 cells (4^3 interior with NG = 3)
+    shape = (10, 10, 10)
+    dx, dy, dz = 1.0 / 4, 1.0 / 5, 1.0 / 6
+    arrs = {}
+    pr = diss_state(shape, 900)
+    arrs["rhs_in"] = pr
+    for c, (recon, lim, riem) in enumerate(DISS_RHS_CASES):
+        srhd_configure(RECON=recon, LIMITER=lim, RIEMANN=riem, N_PASSIVE=2, PASSIVE_OFFSET=15)
+        arrs[f"rhs_{c}"] = srhd_rhs(pr, recon, dx, dy, dz)
+        print(f"[golden] dissipative rhs {recon}/{lim}/{riem}", flush=True)
+    dt = 0.02
+    pr = diss_state(shape, 920)
+    arrs["step_in"] = pr
+    for c, (recon, lim, riem, rk) in enumerate(DISS_STEP_CASES):
+        srhd_configure(RECON=recon, LIMITER=lim, RIEMANN=riem, N_PASSIVE=2, PASSIVE_OFFSET=15)
+        arrs[f"step_{c}"] = (srhd_core.step_ssprk3 if rk == 3 else srhd_core.step_ssprk2)(pr, dx, dy, dz, dt)
+        print(f"[golden] dissipative step {recon} rk{rk}", flush=True)
+    op_in = diss_state(shape, 940, npass=0)
+    arrs["op_in"] = op_in
+    for c, (enabled, cfg) in enumerate(DISS_OP_CASES):
+        full = dict(cfg, DISSIPATION_ENABLED=enabled, PHYSICS="hydro", NG=3)
+        out = dissipation.apply_causal_dissipation(op_in.copy(), 0.01, dx, dy, dz, full)
+        arrs[f"op_{c}"] = out[5:15]
+    _save("dissipation", rhs_cases=np.array(["/".join(c) for c in DISS_RHS_CASES]),
+          step_cases=np.array(["/".join(map(str, c)) for c in DISS_STEP_CASES]),
+          op_cases=np.array([repr(c) for c in DISS_OP_CASES]), dxyz=np.array([dx, dy, dz]), dt=np.array(dt),
+          **arrs)
+
+
+# ---------------------------------------------------------------------------
 # Driver loop (restates the loop order of solvers/srhd3d_mpi_muscl.py:276-326, one rank)
 # ---------------------------------------------------------------------------
 def run_driver(s, pr, nsteps, rng):
@@ -503,6 +558,7 @@ def run_driver(s, pr, nsteps, rng):
         boundary.apply_periodic_yz(pr, ng)
         boundary.apply_outflow_right_x(pr, ng)
         nozzle.apply_nozzle_left_x(pr, dx, dy, dz, s["NY"], s["NZ"], y0, z0, rng, s)
+        pr = dissipation.apply_causal_dissipation(pr, dt, dx, dy, dz, s)
         pr = source_terms.apply_cooling_heating(pr, dt, s)
         pr = source_terms.apply_two_temperature(pr, dt, s)
         pr = chemistry.apply_ion_chemistry(pr, dt, s)
@@ -525,7 +581,7 @@ def driver_settings(**kw):
 
 
 def init_block(s, noise_seed):
-    nv = (9 if s["PHYSICS"] == "rmhd" else 5) + s["N_PASSIVE"]
+    nv = (9 if s["PHYSICS"] == "rmhd" else (15 if s.get("DISSIPATION_ENABLED", False) else 5)) + s["N_PASSIVE"]
     ng = s["NG"]
     pr = np.zeros((nv, s["NX"] + 2 * ng, s["NY"] + 2 * ng, s["NZ"] + 2 * ng))
     pr[0] = s["RHO_AMB"]; pr[4] = s["P_AMB"]
@@ -590,6 +646,23 @@ def gen_driver_sources():
     _save("driver_sources_10", pr0=pr0, out=out, dts=dts, settings=np.array(repr(s)))
 
 
+def gen_driver_dissipation():
+    # SRHD MUSCL-MC + HLLC + SSPRK3 jet with Israel-Stewart dissipation (fields 5..14, a tracer at 15),
+    # bulk, shear and heat terms on, upwind advection of the fields, perturbed nozzle.
+    s = driver_settings(NX=10, NY=9, NZ=8, NG=2, RECON="muscl", LIMITER="mc", RIEMANN="hllc", RK_ORDER=3,
+                        NOZZLE_PERTURB="random", TURB_PAMP=0.05, SHEAR_THICK=0.02, JET_RADIUS=0.3,
+                        DISSIPATION_ENABLED=True, BULK_ZETA=0.01, SHEAR_ETA=0.005, HEAT_KAPPA=0.002,
+                        RELAX_TAU=0.1, DISSIPATION_CAP_FRAC=0.5, DISSIPATION_ADVECT=True,
+                        N_TRACERS=1, TRACER_NOZZLE_VALUES=[1.0], TRACER_AMB_VALUES=[0.0], TRACER_OFFSET=15,
+                        N_PASSIVE=1, PASSIVE_OFFSET=15)
+    pr0 = init_block(s, 5151)
+    t0 = time.time()
+    out, dts = run_driver(s, pr0.copy(), 3, np.random.default_rng(1234))
+    print(f"[golden] srhd dissipation driver {time.time()-t0:.1f}s: |Pi| max {np.abs(out[5]).max():.3e} "
+          f"|pi_xy| max {np.abs(out[9]).max():.3e} |q_x| max {np.abs(out[12]).max():.3e}", flush=True)
+    _save("driver_diss_10", pr0=pr0, out=out, dts=dts, settings=np.array(repr(s)))
+
+
 def gen_driver_32():
     # The 32^3 SRHD jet: MUSCL-minmod + HLLE + SSPRK2, CFL 0.4, 20 steps, +-1% density noise
     # (default_rng(12345)), nozzle radius 4 cells, W_jet = 7.0888; reference periodic y/z boundaries.
@@ -602,7 +675,8 @@ def gen_driver_32():
 
 
 GENS = dict(pointwise=gen_pointwise, rhs=gen_rhs, steps=gen_steps, bc=gen_bc, sources=gen_sources,
-            driver_small=gen_driver_small, driver_sources=gen_driver_sources)
+            driver_small=gen_driver_small, driver_sources=gen_driver_sources,
+            dissipation=gen_dissipation, driver_diss=gen_driver_dissipation)
 
 
 def main():

@@ -8,14 +8,14 @@ from astrofd_amd.domain import Domain, limiter_id, recon_id, riemann_id
 def make_domain(physics, pr_shape, ng, dxyz=(0.1, 0.1, 0.1), gamma=None, p_max=2.0, v_max=0.9999, limiter="mc",
                 recon="muscl", riemann="hlle", rk_order=2, n_passive=0, n_resist=0, resistivity=0.0,
                 precision="fp64", glm_ch=1.0, glm_cp=0.1, bc_y="periodic", bc_z="periodic",
-                two_temperature=False, thermo_offset=None, tei_tau=0.1, c2p_guess="plain"):
+                two_temperature=False, thermo_offset=None, tei_tau=0.1, c2p_guess="plain", passive_offset=None):
     rm = physics == "rmhd"
     nb = 9 if rm else 5
     p = L.Params()
     p.physics = L.AFD_PHYS_RMHD if rm else L.AFD_PHYS_HYDRO
     p.nv = pr_shape[0]
     p.n_passive = n_passive
-    p.passive_offset = nb
+    p.passive_offset = nb if passive_offset is None else passive_offset
     p.nx, p.ny, p.nz = (pr_shape[1] - 2 * ng, pr_shape[2] - 2 * ng, pr_shape[3] - 2 * ng)
     p.ng = ng
     p.dx, p.dy, p.dz = dxyz

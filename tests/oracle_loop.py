@@ -3,8 +3,8 @@
 Follows the time loop of solvers/srhd3d_mpi_muscl.py:276-326 for one rank:
 CFL from max_char_speed (SRHD) or amax = 1 (RMHD, :278-281), BCs before and
 after the step in the order periodic y/z -> outflow +x -> nozzle -x
-(:289-293, :318-322), then the post-step sources: cooling/heating, two-temperature,
-ion chemistry (:324-326).
+(:289-293, :318-322), then the post-step operators: Israel-Stewart dissipation (:323),
+cooling/heating, two-temperature, ion chemistry (:324-326).
 """
 import numpy as np
 
@@ -71,6 +71,7 @@ def run(s, pr, nsteps, seed=1234, b_amb=(0.0, 0.0, 0.0), bc_y="periodic", bc_z="
         bcs(pr)
         pr = O.step(P, pr, dx, dy, dz, dt)
         bcs(pr)
+        O.apply_causal_dissipation(P, pr, dt, dx, dy, dz, s)
         O.apply_cooling_heating(P, pr, dt, s)
         if s.get("TWO_TEMPERATURE", False):
             O.apply_two_temperature(P, pr, dt, s["TEI_TAU"], s["THERMO_OFFSET"])

@@ -165,7 +165,7 @@ def run_gpu_driver(s, pr0, nsteps, seed=1234):
         d.close()
 
 
-@pytest.mark.parametrize("name", ["driver_srhd_10", "driver_rmhd_8", "driver_sources_10", "driver_srhd_32"])
+@pytest.mark.parametrize("name", ["driver_srhd_10", "driver_rmhd_8", "driver_sources_10", "driver_diss_10", "driver_srhd_32"])
 def test_driver_golden(name):
     path = os.path.join(G, name + ".npz")
     if not os.path.exists(path):
@@ -187,6 +187,13 @@ def test_driver_golden(name):
         # elsewhere inside that tolerance, ~1e-6 relative in p of the coldest cells (where p << rho
         # amplifies the relative error of Z); averaged over all cells and variables ~1e-8..1e-7
         assert err <= 1e-5 and l1 <= 1e-7
+    elif s.get("DISSIPATION_ENABLED", False):
+        # the dissipative jet is ill-conditioned in the reference scheme itself: the CPU oracle,
+        # bitwise with the fixture, moves by 6e-6 (max) and 1.3e-6 (L1) over these 3 steps when
+        # its input is perturbed by 1e-15 relative noise (branch switches such as the upwind
+        # direction of the field advection and the con2prim's first-iteration exit); the bar is
+        # five times that spread
+        assert err <= 3e-5 and l1 <= 6e-6
     else:
         assert err <= 1e-8
 

@@ -165,7 +165,7 @@ def test_bc_nozzle_two_temperature():
 
 
 # ---------------------------------------------------------------- driver loops
-@pytest.mark.parametrize("name", ["driver_srhd_10", "driver_rmhd_8", "driver_sources_10", "driver_srhd_32"])
+@pytest.mark.parametrize("name", ["driver_srhd_10", "driver_rmhd_8", "driver_sources_10", "driver_diss_10", "driver_srhd_32"])
 def test_driver_loop(name):
     path = os.path.join(G, name + ".npz")
     if not os.path.exists(path):
@@ -262,3 +262,79 @@ def test_sources_oracle_properties():
     q[5] = C / (C + A)
     O.apply_ion_chemistry(P, q, 0.05, cfg)
     assert np.allclose(q[5][core], (C / (C + A))[core], rtol=1e-13, atol=1e-15)
+
+
+# ---------------------------------------------------------------- Israel-Stewart dissipation
+def test_dissipation_golden():
+    """The SRHD RHS with the dissipative hooks (bulk pressure on MUSCL face pressures, Pi/pi/q
+    fluxes once per MUSCL face, twice per PPM face, none in WENO), the step carrying fields
+    5..14 with the passives at 15, and core/dissipation.py's operator, bitwise."""
+    f = load("dissipation")
+    dx, dy, dz = f["dxyz"]
+    pr = f["rhs_in"]
+    for c, name in enumerate(f["rhs_cases"]):
+        recon, lim, riem = str(name).split("/")
+        P = O.make_params("hydro", pr.shape[1:], nv=pr.shape[0], ng=3, p_max=2.0, v_max=0.9999, limiter=lim,
+                          recon=recon, riemann=riem, n_passive=2, passive_offset=15)
+        assert_bitwise(O.rhs(P, pr, dx, dy, dz), f[f"rhs_{c}"], f"dissipative rhs {name}")
+    pr = f["step_in"]
+    for c, name in enumerate(f["step_cases"]):
+        recon, lim, riem, rk = str(name).split("/")
+        P = O.make_params("hydro", pr.shape[1:], nv=pr.shape[0], ng=3, p_max=2.0, v_max=0.9999, limiter=lim,
+                          recon=recon, riemann=riem, rk_order=int(rk), n_passive=2, passive_offset=15)
+        assert_bitwise(O.step(P, pr, dx, dy, dz, float(f["dt"])), f[f"step_{c}"], f"dissipative step {name}")
+    for c, s in enumerate(f["op_cases"]):
+        enabled, cfg = ast.literal_eval(str(s))
+        pr = f["op_in"].copy()
+        full = dict(cfg, DISSIPATION_ENABLED=enabled, PHYSICS="hydro")
+        P = O.make_params("hydro", pr.shape[1:], nv=pr.shape[0], ng=3)
+        O.apply_causal_dissipation(P, pr, 0.01, dx, dy, dz, full)
+        assert_bitwise(pr[5:15], f[f"op_{c}"], f"dissipation operator {c}")
+
+
+def test_dissipation_oracle_properties():
+    """What the mathematics fixes: without dissipative fields (nv < 15, PASSIVE_OFFSET = 5) the
+    hooks vanish; uniform fields with v = 0 give a flux divergence of zero whatever Pi, pi, q are;
+    the relaxation with tau -> dt/(large) reaches the Navier-Stokes targets (Pi = -zeta div v on a
+    linear velocity field, whose centred differences are exact); the shear result is traceless."""
+    rng = np.random.default_rng(3)
+    shape = (10, 10, 10)
+    pr = np.zeros((15,) + shape)
+    pr[0], pr[4] = 1.0, 0.1
+    pr[5:15] = rng.uniform(-0.01, 0.01, (10, 1, 1, 1))  # uniform per field
+    P = O.make_params("hydro", shape, nv=15, ng=3, p_max=2.0, v_max=0.9999, passive_offset=15)
+    for recon in ("muscl", "ppm", "weno"):
+        P.recon_id = {"muscl": 0, "ppm": 1, "weno": 2}[recon]
+        r = O.rhs(P, pr, 0.1, 0.1, 0.1)
+        assert np.abs(r).max() == 0.0, recon
+    # linear velocity field v = (a x, b y, c z): div v = a + b + c exactly on the grid
+    x = (np.arange(10) - 2.5) * 0.1
+    a, b, cz = 0.3, -0.2, 0.05
+    pr[1] = a * x[:, None, None]
+    pr[2] = b * x[None, :, None]
+    pr[3] = cz * x[None, None, :]
+    q = pr.copy()
+    O.apply_causal_dissipation(P, q, 1.0, 0.1, 0.1, 0.1, dict(
+        DISSIPATION_ENABLED=True, PHYSICS="hydro", BULK_ZETA=0.2, SHEAR_ETA=0.1, RELAX_TAU=1e-12,
+        DISSIPATION_ADVECT=False, P_MAX=1e9))
+    core = (slice(3, -3),) * 3
+    assert np.allclose(q[5][core], -0.2 * (a + b + cz), rtol=1e-9, atol=1e-15)
+    div = a + b + cz
+    assert np.allclose(q[6][core], -2 * 0.1 * (a - div / 3), rtol=1e-9, atol=1e-15)
+    assert np.allclose(q[6][core] + q[7][core] + q[8][core], 0.0, atol=1e-15)
+    # no off-diagonal shear: only the initial value survives, damped by tau / dt = 1e-12
+    assert np.allclose(q[9:12][(slice(None),) + core], 0.0, atol=1e-13)
+
+
+def test_dissipative_driver_conditioning():
+    """The spread the GPU bar of driver_diss_10 rests on: the oracle's own response to 1e-15
+    relative input noise over the fixture's 3 steps stays below 1e-5 (max) and 2e-6 (L1)."""
+    f = load("driver_diss_10")
+    s = ast.literal_eval(str(f["settings"]))
+    pr0 = f["pr0"]
+    out, _ = oracle_loop.run(s, pr0.copy(), 3)
+    pert = pr0 * (1.0 + 1e-15 * np.random.default_rng(0).standard_normal(pr0.shape))
+    out2, _ = oracle_loop.run(s, pert, 3)
+    err = max(float(np.abs(out2[v] - out[v]).max() / np.abs(out[v]).max()) for v in range(out.shape[0]))
+    l1 = max(float(np.abs(out2[v] - out[v]).sum() / np.abs(out[v]).sum()) for v in range(out.shape[0]))
+    assert 1e-8 < err < 1e-5 and l1 < 2e-6
