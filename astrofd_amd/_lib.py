@@ -36,6 +36,7 @@ class Params(ctypes.Structure):
         ("precision", _i32), ("c2p_guess", _i32), ("bc_y", _i32), ("bc_z", _i32),
         ("two_temperature", _i32), ("thermo_offset", _i32), ("tei_tau", _dbl),
         ("device", _i32),
+        ("gr_metric", _i32), ("gr_orthonormal", _i32), ("gr_x0", _i32), ("gr_mass", _dbl), ("gr_gamma", _dbl),
     ]
 
 

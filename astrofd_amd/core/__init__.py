@@ -1,5 +1,6 @@
 """Reference-named operator modules (core/srhd_core.py, core/rmhd_core.py, core/boundary.py,
-core/nozzle.py, core/source_terms.py, core/chemistry.py, core/dissipation.py) on the MI355X library.
+core/nozzle.py, core/source_terms.py, core/chemistry.py, core/dissipation.py, core/grhd_core.py,
+core/grmhd_core.py) on the MI355X library.
 
 Same names, arguments and return conventions as the reference: ``configure(params)`` first;
 ``step_ssprk2/3`` and ``compute_rhs_*`` take and return numpy arrays (the input is not

@@ -26,12 +26,14 @@ def settings(physics):
     return s
 
 
-def domain(physics, full_shape, dxyz=(1.0, 1.0, 1.0), rk_order=2, ng=None, gamma=None):
-    """A context for arrays of shape (nv, NX+2NG, NY+2NG, NZ+2NG)."""
+def domain(physics, full_shape, dxyz=(1.0, 1.0, 1.0), rk_order=2, ng=None, gamma=None, gr=None):
+    """A context for arrays of shape (nv, NX+2NG, NY+2NG, NZ+2NG).  `gr`: a dict of GR_METRIC,
+    GR_MASS, ORTHONORMAL_FLUX and the global x offset "offs_x" for the GR wrappers."""
     s = settings(physics)
     ng = int(s["NG"] if ng is None else ng)
     nv, fx, fy, fz = (int(v) for v in full_shape)
-    key = (physics, nv, fx, fy, fz, ng, tuple(float(d) for d in dxyz), int(rk_order), gamma)
+    grkey = tuple(sorted(gr.items())) if gr else None
+    key = (physics, nv, fx, fy, fz, ng, tuple(float(d) for d in dxyz), int(rk_order), gamma, grkey)
     d = _CACHE.get(key)
     if d is None:
         s2 = dict(s, PHYSICS=physics)
@@ -44,8 +46,15 @@ def domain(physics, full_shape, dxyz=(1.0, 1.0, 1.0), rk_order=2, ng=None, gamma
         over["n_passive"] = nv - po
         if gamma is not None:
             over["gamma"] = float(gamma)
-        # NX..NZ only size the settings-derived dx; the overrides set the grid
+        # NX..NZ and Lx..Lz only size the settings-derived dx (the reference's configure() reads
+        # neither); the overrides set the grid
         s2.update(NX=max(fx - 2 * ng, 1), NY=max(fy - 2 * ng, 1), NZ=max(fz - 2 * ng, 1))
+        for key in ("Lx", "Ly", "Lz"):
+            s2.setdefault(key, 1.0)
+        if gr:
+            s2.update(PHYSICS="grmhd" if physics == "rmhd" else "grhd", GR_METRIC=gr["GR_METRIC"],
+                      GR_MASS=gr["GR_MASS"], ORTHONORMAL_FLUX=gr["ORTHONORMAL_FLUX"])
+            over["gr_x0"] = int(gr.get("offs_x", 0))
         d = Domain(params_from_settings(s2, **over))
         _CACHE[key] = d
     return d

@@ -33,6 +33,7 @@ struct afd_ctx {
     int nb = 5;
     bool carried = false, passive_flux = false;
     int diss = 0;  // FaceArgs::diss of the SRHD faces
+    double *Puse = nullptr;  // GR ORTHONORMAL_FLUX: the stage input with rescaled velocities
     size_t fsize = 8;
     hipStream_t stream = nullptr;
     double *P[3] = {nullptr, nullptr, nullptr};
@@ -198,6 +199,8 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
         return fail(AFD_ERR_INVALID, "thermo_offset outside the passive block");
     if (!(p.dx > 0 && p.dy > 0 && p.dz > 0)) return fail(AFD_ERR_INVALID, "dx, dy, dz must be > 0");
     if (p.n_resist < 0 || p.n_resist > 2) return fail(AFD_ERR_INVALID, "n_resist must be 0, 1 or 2");
+    if (p.gr_metric < 0 || p.gr_metric > 3) return fail(AFD_ERR_INVALID, "gr_metric must be in [0, 3]");
+    if (p.gr_metric != 0 && !(p.gr_gamma > 1.0)) return fail(AFD_ERR_INVALID, "gr_gamma must be > 1");
 
     HIPCHK(hipSetDevice(p.device));
     afd_ctx *c = new afd_ctx();
@@ -244,6 +247,7 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
     } while (0)
     size_t N = (size_t)g.ncell;
     for (int b = 0; b < 3; ++b) TRY(dalloc(c, (void **)&c->P[b], N * p.nv * 8));
+    if (p.gr_orthonormal && (p.gr_metric == 1 || p.gr_metric == 2)) TRY(dalloc(c, (void **)&c->Puse, N * p.nv * 8));
     TRY(dalloc(c, (void **)&c->U0, N * nb * 8));
     TRY(dalloc(c, (void **)&c->Uc, N * nb * 8));
     TRY(dalloc(c, &c->Fx, N * nb * c->fsize));
@@ -536,6 +540,13 @@ static FaceLaunch face_launch(afd_ctx *c, const double *P) {
 }
 
 static int run_faces(afd_ctx *c, const double *P) {
+    if (c->Puse) {  // GR: the fluxes see the rescaled velocities (core/grhd_core.py:22-45)
+        GrRescaleArgs r{P, c->Puse, c->g, c->p.ng, c->p.gr_x0, c->p.gr_mass, c->p.dx, c->p.dy, c->p.dz};
+        Timed t(c, 4);
+        hipLaunchKernelGGL(k_gr_rescale, grid1d(c->g.ncell), dim3(BLOCK), 0, c->stream, r);
+        HIPCHK(hipGetLastError());
+        P = c->Puse;
+    }
     FaceLaunch l = face_launch(c, P);
     void *Fd[3] = {c->Fx, c->Fy, c->Fz};
     const int order_carried[3] = {2, 0, 1}, order_plain[3] = {0, 1, 2};
@@ -609,6 +620,11 @@ static UpdLaunch upd_launch(afd_ctx *c, const double *Pin, const double *P0, dou
     a.guess = c->p.c2p_guess;
     a.queue = c->p.c2p_guess != AFD_C2P_PLAIN ? c->c2p_queue : nullptr;
     a.qcount = c->p.c2p_guess != AFD_C2P_PLAIN ? c->c2p_qcount : nullptr;
+    a.ng = c->p.ng;
+    a.gr = c->p.gr_metric == 0 ? 0 : (c->p.gr_metric == 1 ? 1 : 2);
+    a.gr_x0 = c->p.gr_x0;
+    a.gr_mass = c->p.gr_mass;
+    a.gr_gfac = c->p.gr_metric == 0 ? 0.0 : c->p.gr_gamma / (c->p.gr_gamma - 1.0);
     a.qcap = c->c2p_qcap;
     return l;
 }

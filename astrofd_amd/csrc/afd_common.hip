@@ -320,6 +320,23 @@ __global__ __launch_bounds__(BLOCK) void k_diss_store(DissArgs a, double *P) {
     for (int m = 0; m < 10; ++m) P[(5 + m) * n + c] = a.S[m * n + c];
 }
 
+__global__ __launch_bounds__(BLOCK) void k_gr_rescale(GrRescaleArgs a) {
+#pragma clang fp contract(off)
+    const Grid &g = a.g;
+    const long c = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= g.ncell) return;
+    const long n = g.ncell;
+    for (int v = 0; v < g.nv; ++v) a.Q[v * n + c] = a.P[v * n + c];
+    const int k = (int)(c % g.nz);
+    const long t = c / g.nz;
+    const int j = (int)(t % g.ny), i = (int)(t / g.ny);
+    if (i < a.ng || i >= g.nx - a.ng || j < a.ng || j >= g.ny - a.ng || k < a.ng || k >= g.nz - a.ng) return;
+    double X[3];
+    gr_xyz(a.x0, a.ng, i, j, k, a.dx, a.dy, a.dz, X);
+    const double ax = pymax(gr_ks_alpha(X[0], X[1], X[2], a.mass), 1e-12);
+    for (int v = 1; v <= 3; ++v) a.Q[v * n + c] = a.P[v * n + c] / ax;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_max_speed(MaxSpeedArgs a) {
     const Grid &g = a.g;
     __shared__ double red[BLOCK];

@@ -367,6 +367,8 @@ template <typename F> struct UpdArgs {
     int *queue;                   // learned guess: cells the judge rejected, QUEUE_SLOTS regions
     unsigned *qcount;             //   of qcap entries each, filled per workgroup slot (or null)
     long qcap;
+    int ng, gr, gr_x0;            // GR sources: 0 none, 1 isotropic-Schwarzschild lapse, 2 Kerr-Schild
+    double gr_mass, gr_gfac;      // GR_MASS; Gamma / (Gamma - 1) of the sources' enthalpy
 };
 
 // Rejected cells of the learned guess are queued per workgroup slot (blockIdx % QUEUE_SLOTS),
@@ -412,12 +414,80 @@ AFD_D void wave_add_stats(unsigned long long *stats, const C2PStats &st) {
 // Flux divergence and sources of one cell, in the reference's order of accumulation:
 // x, y, z differences (core/srhd_core.py:662, 779, 922), then GLM damping and the resistive
 // Laplacian of B (core/rmhd_core.py:1264-1275), n_resist times.
-template <int NB, typename F> AFD_D void cell_rhs(const UpdArgs<F> &a, long c, bool in, double *rhs) {
+// core/gr_metric.py:6-16 and :18-38: d ln(alpha) / dx^i of the isotropic Schwarzschild lapse
+// (metric 1) or the Kerr-Schild lapse (otherwise; the Kerr-Schild helper ignores the spin).
+AFD_D void gr_lapse_grad(int metric, double x, double y, double z, double M, double *d) {
+#pragma clang fp contract(off)
+    const double r = sqrt(x * x + y * y + z * z) + 1e-12;
+    if (metric == 1) {
+        const double a = M / (2.0 * r), denom = 1.0 - a * a;
+        const double dr = denom == 0.0 ? 0.0 : (2.0 * a) / (r * denom);
+        d[0] = dr * (x / r);
+        d[1] = dr * (y / r);
+        d[2] = dr * (z / r);
+    } else {
+        const double H = M / r, dr = H / (r * (1.0 + 2.0 * H));
+        d[0] = dr * (x / r);
+        d[1] = dr * (y / r);
+        d[2] = dr * (z / r);
+    }
+}
+
+// core/gr_metric.py:18-22: the Kerr-Schild lapse (the flux-velocity rescale of every metric)
+AFD_D double gr_ks_alpha(double x, double y, double z, double M) {
+#pragma clang fp contract(off)
+    const double r = sqrt(x * x + y * y + z * z) + 1e-12;
+    return 1.0 / sqrt(1.0 + 2.0 * (M / r));
+}
+
+// Cell-centre coordinates of cell (i, j, k): core/grhd_core.py:26-30, 51-55
+AFD_D void gr_xyz(int x0, int ng, int i, int j, int k, double dx, double dy, double dz, double *X) {
+#pragma clang fp contract(off)
+    X[0] = ((double)(x0 + (i - ng)) + 0.5) * dx;
+    X[1] = ((double)(j - ng) + 0.5) * dy;
+    X[2] = ((double)(k - ng) + 0.5) * dz;
+}
+
+// core/grhd_core.py:50-73 / core/grmhd_core.py:50-75: the lapse-gradient sources of an interior
+// cell from the stage input's (unscaled) primitives.  The momentum source is (rho h W^2 [+ B^2] + p)
+// d ln(alpha), the energy source S . d ln(alpha), with S from the flat p2c (the SRHD one keeps the
+// definition-time 5/3; h takes the configured Gamma).
+template <int NB, typename F> AFD_D void gr_source(const UpdArgs<F> &a, long c, double *rhs) {
 #pragma clang fp contract(off)
     const Grid &g = a.g;
+    const int k = (int)(c % g.nz);
+    const long t = c / g.nz;
+    const int j = (int)(t % g.ny), i = (int)(t / g.ny);
+    if (i < a.ng || i >= g.nx - a.ng || j < a.ng || j >= g.ny - a.ng || k < a.ng || k >= g.nz - a.ng) return;
+    double X[3], d[3], q[NB], U[NB];
+    gr_xyz(a.gr_x0, a.ng, i, j, k, a.dx, a.dy, a.dz, X);
+    gr_lapse_grad(a.gr, X[0], X[1], X[2], a.gr_mass, d);
 #pragma unroll
-    for (int v = 0; v < NB; ++v) rhs[v] = 0.0;
-    if (!in) return;
+    for (int v = 0; v < NB; ++v) q[v] = a.Pin[v * g.ncell + c];
+    const double rho = q[0], p = q[4];
+    double v2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    if (v2 >= 1.0) v2 = 1.0 - 1e-14;
+    const double W = 1.0 / sqrt(1.0 - v2);
+    const double h = 1.0 + a.gr_gfac * p / pymax(rho, SMALL);
+    double w;
+    if constexpr (NB == 9) {
+        const double B2 = q[5] * q[5] + q[6] * q[6] + q[7] * q[7];
+        w = rho * h * W * W + B2 + p;
+        rmhd_p2c_cell(a.gfac, a.p_max, a.v_max, q, U);
+    } else {
+        w = rho * h * W * W + p;
+        srhd_p2c_cell(a.gfac, q, U);
+    }
+    rhs[1] += w * d[0];
+    rhs[2] += w * d[1];
+    rhs[3] += w * d[2];
+    rhs[4] += U[1] * d[0] + U[2] * d[1] + U[3] * d[2];
+}
+
+// The flat RHS of a cell in the flux ranges: flux divergence, GLM and resistive sources.
+template <int NB, typename F> AFD_D void cell_rhs_flat(const UpdArgs<F> &a, long c, double *rhs) {
+#pragma clang fp contract(off)
+    const Grid &g = a.g;
     if constexpr (sizeof(F) == 8) {  // FP64 path: the reference's divisions
 #pragma unroll
         for (int v = 0; v < NB; ++v) {
@@ -451,6 +521,15 @@ template <int NB, typename F> AFD_D void cell_rhs(const UpdArgs<F> &a, long c, b
             }
         }
     }
+}
+
+// The RHS of one cell: the flat RHS in the flux ranges (`in`), plus the GR sources on interior cells.
+template <int NB, typename F> AFD_D void cell_rhs(const UpdArgs<F> &a, long c, bool in, double *rhs) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int v = 0; v < NB; ++v) rhs[v] = 0.0;
+    if (in) cell_rhs_flat<NB, F>(a, c, rhs);
+    if (a.gr) gr_source<NB, F>(a, c, rhs);
 }
 
 AFD_D bool in_range(const Grid &g, long c) {
@@ -677,6 +756,17 @@ struct DissArgs {
     double zeta, eta, kappa, tau_bulk, tau_shear, tau_heat, cap;
 };
 __global__ void k_diss(DissArgs a);
+
+// core/grhd_core.py:22-38, core/grmhd_core.py:22-38 (ORTHONORMAL_FLUX): Q = P with the interior
+// velocities divided by max(alpha_KS, 1e-12); the face kernels then read Q.  One thread per cell.
+struct GrRescaleArgs {
+    const double *P;
+    double *Q;
+    Grid g;
+    int ng, x0;
+    double mass, dx, dy, dz;
+};
+__global__ void k_gr_rescale(GrRescaleArgs a);
 __global__ void k_diss_store(DissArgs a, double *P);
 
 // ------------------------------------------------------------------ reductions

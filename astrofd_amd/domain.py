@@ -24,7 +24,7 @@ def recon_id(name):
 
 def riemann_id(physics, name):
     name = str(name).lower()
-    if physics == "rmhd":  # core/rmhd_core.py:42-46: "hllc" falls through to HLLE
+    if physics in ("rmhd", "grmhd"):  # core/rmhd_core.py:42-46: "hllc" falls through to HLLE
         return 2 if name == "hlld_full" else (1 if name == "hlld" else 0)
     return 1 if name == "hllc" else 0  # core/srhd_core.py:38-39
 
@@ -41,9 +41,9 @@ def n_resist_for(s, recon):
 
 def params_from_settings(s, device=0, **override):
     phys = str(s.get("PHYSICS", "hydro")).lower()
-    if phys not in ("hydro", "rmhd"):
-        raise NotImplementedError(f"PHYSICS={phys!r}: only the SRHD (hydro) and RMHD step are on this path")
-    rm = phys == "rmhd"
+    if phys not in ("hydro", "rmhd", "grhd", "grmhd"):
+        raise NotImplementedError(f"PHYSICS={phys!r}: the path runs hydro, rmhd, grhd and grmhd")
+    rm = phys in ("rmhd", "grmhd")
     nb = 9 if rm else 5
     rc = recon_id(s.get("RECON", "muscl"))
     p = L.Params()
@@ -77,9 +77,24 @@ def params_from_settings(s, device=0, **override):
     p.thermo_offset = int(s.get("THERMO_OFFSET", nb))
     p.tei_tau = float(s.get("TEI_TAU", 0.5))
     p.device = device
+    if phys in ("grhd", "grmhd"):  # core/grhd_core.py:13-18, core/grmhd_core.py:13-18
+        p.gr_metric = gr_metric_id(s.get("GR_METRIC", "minkowski"))
+        p.gr_orthonormal = int(bool(s.get("ORTHONORMAL_FLUX", True)))
+        p.gr_mass = float(s.get("GR_MASS", 1.0))
+        # the sources' enthalpy: srhd_core.GAMMA (the configured one) / rmhd_core.GAMMA
+        p.gr_gamma = float(s.get("GAMMA", 5.0 / 3.0)) if not rm else p.gamma
     for k, v in override.items():
         setattr(p, k, v)
     return p
+
+
+GR_METRICS = {"minkowski": 0, "schwarzschild": 1, "kerr-schild": 2}
+
+
+def gr_metric_id(name):
+    """core/grhd_core.py:23, 47, 56: Minkowski skips the wrapper; Schwarzschild and Kerr-Schild
+    rescale the flux velocities; any other name takes the Kerr-Schild sources only (3)."""
+    return GR_METRICS.get(str(name).lower(), 3)
 
 
 PROFILES = {"top_hat": 0, "taper": 1, "smooth": 1, "parabolic": 2, "parabola": 2}

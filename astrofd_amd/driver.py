@@ -90,7 +90,7 @@ def output_fields(s, pr):
         for m in range(int(s["N_CHEM"])):
             f[names[m] if m < len(names) else f"chem{m}"] = pr[int(s["CHEM_OFFSET"]) + m]
     base = dict(rho=pr[0], vx=pr[1], vy=pr[2], vz=pr[3], p=pr[4])
-    if s["PHYSICS"] == "rmhd":
+    if s["PHYSICS"] in ("rmhd", "grmhd"):
         base.update(Bx=pr[5], By=pr[6], Bz=pr[7], psi=pr[8])
     elif int(s.get("TRACER_OFFSET", 5)) > 5:  # the dissipative fields (:362-372)
         for m, name in enumerate(("pi", "pixx", "piyy", "pizz", "pixy", "pixz", "piyz", "qx", "qy", "qz")):
@@ -123,7 +123,7 @@ def write_diagnostics(d, s, run_dir, step, t, dt, amax, dx):
         for i in range(cl.shape[1]):
             f.write(f"{step},{t:.8e},{i},{(i + 0.5) * dx:.8e},{cl[0, i]:.8e},{cl[1, i]:.8e},{cl[2, i]:.8e},"
                     f"{cl[3, i]:.8e}\n")
-    if s["PHYSICS"] == "rmhd":
+    if s["PHYSICS"] in ("rmhd", "grmhd"):
         _append_csv(os.path.join(run_dir, "divb.csv"), "step,time,divB_max,divB_rms,divB_mean",
                     f"{step},{t:.8e},{g['divb_max']:.6e},{g['divb_rms']:.6e},{g['divb_mean']:.6e}")
     return g
@@ -163,7 +163,7 @@ def run(s, log=print):
     try:
         d.set_prim(pr)
         d.set_nozzle(s)
-        if s["PHYSICS"] == "rmhd" and s["C2P_GUESS"] == "learned":
+        if s["PHYSICS"] in ("rmhd", "grmhd") and s["C2P_GUESS"] == "learned":
             d.set_mlp_weights(load_mlp(s))
         if not restart_path:
             d.apply_bcs(rng)
@@ -172,7 +172,7 @@ def run(s, log=print):
             log("[jit] RHS kernel compiled and first call done.")
 
         T_END, max_steps = float(s["T_END"]), int(s.get("MAX_STEPS", 0) or 0)
-        hydro = s["PHYSICS"] == "hydro"
+        hydro = s["PHYSICS"] in ("hydro", "grhd")
         t0 = time.perf_counter()
         while t < T_END and (max_steps <= 0 or step < max_steps):
             amax = d.max_char_speed() if hydro else 1.0
@@ -195,7 +195,7 @@ def run(s, log=print):
             if step % int(s["OUT_EVERY"]) == 0 or abs(t - T_END) < 1e-14:
                 pr = d.get_prim()
                 fname = os.path.join(run_dir, f"jet3d_rank0000_step{step:06d}.npz")
-                kind = "3D RMHD/GRMHD block (with ghosts)." if s["PHYSICS"] == "rmhd" else \
+                kind = "3D RMHD/GRMHD block (with ghosts)." if s["PHYSICS"] in ("rmhd", "grmhd") else \
                     "3D SRHD/GRHD block (with ghosts)."
                 np.savez(fname, **output_fields(s, pr), meta=np.array([dx, dy, dz, t], dtype=np.float64),
                          comment=kind)

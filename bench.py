@@ -84,7 +84,7 @@ def make_stepper(d, s, rng):
 
     def one_step():
         # the reference driver's step: CFL (SRHD only; RMHD uses amax = 1), fills, SSPRK, fills, sources
-        amax = d.max_char_speed() if s["PHYSICS"] == "hydro" else 1.0
+        amax = d.max_char_speed() if s["PHYSICS"] in ("hydro", "grhd") else 1.0
         dt = s["CFL"] * dx / max(amax, 1e-12)
         d.apply_bcs(rng)
         d.step(dt)

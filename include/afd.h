@@ -69,6 +69,14 @@ typedef struct afd_params {
     int32_t two_temperature, thermo_offset;
     double tei_tau;
     int32_t device;           /* HIP device ordinal */
+    /* GRHD / GRMHD on a fixed metric (core/grhd_core.py, core/grmhd_core.py; PHYSICS "grhd" or
+     * "grmhd" runs the SRHD or RMHD step with these set; zeros = flat space, the reference's
+     * "hydro" / "rmhd"). */
+    int32_t gr_metric;        /* 0 minkowski, 1 schwarzschild, 2 kerr-schild, 3 any other name */
+    int32_t gr_orthonormal;   /* ORTHONORMAL_FLUX: fluxes see v / alpha_KS (metrics 1 and 2) */
+    int32_t gr_x0;            /* global x index of the first interior cell (0 on one GPU) */
+    double gr_mass;           /* GR_MASS */
+    double gr_gamma;          /* Gamma of the sources' enthalpy (GRHD: the configured GAMMA) */
 } afd_params;
 
 /* Nozzle inflow on the -x ghost planes (core/nozzle.py:27-92).  The library's host code
@@ -142,7 +150,9 @@ int afd_apply_nozzle_left_x(afd_ctx *ctx, const double *draws, int64_t ndraws); 
 int afd_max_char_speed(afd_ctx *ctx, double *amax);
 
 /* One SSPRK2/3 step of the current state, device resident:
- * core/srhd_core.py:1286-1400, core/rmhd_core.py:1559-1677 (step_ssprk2/3). */
+ * core/srhd_core.py:1286-1400, core/rmhd_core.py:1559-1677 (step_ssprk2/3); with gr_metric set,
+ * core/grhd_core.py:77-166 and core/grmhd_core.py:78-183, whose RHS (:20-75 / :20-77) wraps the
+ * flat one. */
 int afd_step(afd_ctx *ctx, double dt);
 
 /* RHS of the current state, (nb + n_passive) x full cells, copied to host:

@@ -29,7 +29,18 @@ class Params(ctypes.Structure):
         ("limiter_id", ctypes.c_int), ("recon_id", ctypes.c_int), ("riemann_id", ctypes.c_int),
         ("rk_order", ctypes.c_int), ("n_passive", ctypes.c_int), ("passive_offset", ctypes.c_int),
         ("resistive", ctypes.c_int), ("resistivity", ctypes.c_double),
+        ("gr_metric", ctypes.c_int), ("gr_orthonormal", ctypes.c_int), ("gr_x0", ctypes.c_int),
+        ("gr_mass", ctypes.c_double), ("gr_gamma", ctypes.c_double),
     ]
+
+
+GR_METRICS = {"minkowski": 0, "schwarzschild": 1, "kerr-schild": 2}
+
+
+def gr_metric_id(name):
+    """core/grhd_core.py:23, 47, 56: Minkowski skips everything; Schwarzschild and Kerr-Schild
+    rescale the flux velocities; any other name takes the Kerr-Schild sources only."""
+    return GR_METRICS.get(str(name).lower(), 3)
 
 
 class Nozzle(ctypes.Structure):
@@ -128,21 +139,27 @@ RECONS = {"muscl": 0, "ppm": 1, "weno": 2}
 
 def riemann_id(physics, name):
     name = str(name).lower()
-    if physics == "rmhd":
+    if physics in ("rmhd", "grmhd"):
         return 2 if name == "hlld_full" else (1 if name == "hlld" else 0)
     return 1 if name == "hllc" else 0
 
 
 def make_params(physics="hydro", shape=(1, 1, 1), nv=None, ng=2, gamma=None, p_max=1.0, v_max=0.999,
                 glm_ch=1.0, glm_cp=0.1, limiter="mc", recon="muscl", riemann="hlle", rk_order=2,
-                n_passive=0, passive_offset=None, resistive=False, resistivity=0.0):
+                n_passive=0, passive_offset=None, resistive=False, resistivity=0.0, gr_metric="minkowski",
+                gr_mass=1.0, orthonormal_flux=True, gr_gamma=None, gr_x0=0):
     """Parameters as the reference's configure() leaves them (core/srhd_core.py:18-41,
     core/rmhd_core.py:21-50).  SRHD always computes with Gamma = 5/3 (Q5)."""
-    rm = physics == "rmhd"
+    rm = physics in ("rmhd", "grmhd")
     nb = 9 if rm else 5
     if not rm or gamma is None:
         gamma = 5.0 / 3.0
-    return Params(physics=1 if rm else 0, nv=nv if nv is not None else nb + n_passive,
+    if rm or gr_gamma is None:
+        # GRMHD: rmhd_core.GAMMA in the source's enthalpy; GRHD: srhd_core.GAMMA, the configured
+        # Gamma (pass it as gr_gamma), while its p2c keeps the definition-time 5/3
+        gr_gamma = gamma if rm else 5.0 / 3.0
+    return Params(gr_metric=gr_metric_id(gr_metric), gr_orthonormal=int(bool(orthonormal_flux)), gr_x0=int(gr_x0),
+                  gr_mass=float(gr_mass), gr_gamma=float(gr_gamma), physics=1 if rm else 0, nv=nv if nv is not None else nb + n_passive,
                   nx=shape[0], ny=shape[1], nz=shape[2], ng=ng, gamma=gamma, p_max=p_max, v_max=v_max,
                   glm_ch=glm_ch, glm_cp=glm_cp, limiter_id=LIMITERS.get(limiter, 0), recon_id=RECONS.get(recon, 0),
                   riemann_id=riemann_id(physics, riemann), rk_order=rk_order, n_passive=n_passive,

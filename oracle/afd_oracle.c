@@ -1136,20 +1136,108 @@ static void rmhd_rhs_ppm_weno(const orc_params *P, const double *pr, double *rhs
 /* =====================================================================
  * Whole-array operators
  * ===================================================================== */
+/* core/gr_metric.py:6-16 (schwarzschild_iso_lapse_and_grad): d ln(alpha) / dx^i */
+static void gr_iso_grad(double x, double y, double z, double M, double *d) {
+    double r = sqrt(x * x + y * y + z * z) + 1e-12;
+    double a = M / (2.0 * r);
+    double denom = (1.0 - a * a);
+    double dlnadr = denom == 0.0 ? 0.0 : (2.0 * a) / (r * denom);
+    d[0] = dlnadr * (x / r);
+    d[1] = dlnadr * (y / r);
+    d[2] = dlnadr * (z / r);
+}
+
+/* core/gr_metric.py:18-38 (schwarzschild_ks_lapse_shift_and_grad; the Kerr-Schild helper ignores
+ * the spin): returns the lapse, and d ln(alpha) / dx^i into d when d != NULL */
+static double gr_ks(double x, double y, double z, double M, double *d) {
+    double r = sqrt(x * x + y * y + z * z) + 1e-12;
+    double H = M / r;
+    double alpha = 1.0 / sqrt(1.0 + 2.0 * H);
+    if (d) {
+        double lx = x / r, ly = y / r, lz = z / r;
+        double dlnadr = H / (r * (1.0 + 2.0 * H));
+        d[0] = dlnadr * lx;
+        d[1] = dlnadr * ly;
+        d[2] = dlnadr * lz;
+    }
+    return alpha;
+}
+
+static inline double gr_x(const orc_params *P, int i, double dx) { return ((double)(P->gr_x0 + (i - P->ng)) + 0.5) * dx; }
+
+/* core/grhd_core.py:50-73, core/grmhd_core.py:50-75: the fixed-metric source terms from the
+ * original primitives (not the rescaled ones), interior cells */
+static void gr_sources(const orc_params *P, const double *pr, double *rhs, double dx, double dy, double dz) {
+    const int ng = P->ng, rm = P->physics == 1;
+#pragma omp parallel for schedule(static)
+    for (int i = ng; i < NXF - ng; ++i)
+        for (int j = ng; j < NYF - ng; ++j)
+            for (int k = ng; k < NZF - ng; ++k) {
+                double x = gr_x(P, i, dx), y = ((double)(j - ng) + 0.5) * dy, z = ((double)(k - ng) + 0.5) * dz;
+                double d[3];
+                if (P->gr_metric == 1) gr_iso_grad(x, y, z, P->gr_mass, d);
+                else (void)gr_ks(x, y, z, P->gr_mass, d);
+                double q[9], U[9];
+                for (int v = 0; v < (rm ? 9 : 5); ++v) q[v] = PR(v, i, j, k);
+                double rho = q[0], vx = q[1], vy = q[2], vz = q[3], p = q[4];
+                double v2 = vx * vx + vy * vy + vz * vz;
+                if (v2 >= 1.0) v2 = 1.0 - 1e-14;
+                double W = 1.0 / sqrt(1.0 - v2);
+                double G = P->gr_gamma;
+                double h = 1.0 + G / (G - 1.0) * p / pymax(rho, SMALL);
+                double w;
+                if (rm) {
+                    double B2 = q[5] * q[5] + q[6] * q[6] + q[7] * q[7];
+                    w = rho * h * W * W + B2 + p;
+                    rmhd_p2c(P->gamma, P->p_max, P->v_max, q, U);
+                } else {
+                    w = rho * h * W * W + p;
+                    srhd_p2c(P->gamma, q, U);
+                }
+                const size_t N = (size_t)NXF * NYF * NZF, c = IX(0, i, j, k);
+                rhs[1 * N + c] += w * d[0];
+                rhs[2 * N + c] += w * d[1];
+                rhs[3 * N + c] += w * d[2];
+                rhs[4 * N + c] += U[1] * d[0] + U[2] * d[1] + U[3] * d[2];
+            }
+}
+
+/* The RHS of the step: core/srhd_core.py and core/rmhd_core.py compute_rhs_*, wrapped for GR as
+ * core/grhd_core.py:20-75 / core/grmhd_core.py:20-77: with ORTHONORMAL_FLUX and a Schwarzschild or
+ * Kerr-Schild metric the fluxes see the interior velocities divided by the Kerr-Schild lapse
+ * (max(alpha, 1e-12)); any metric but Minkowski adds the lapse-gradient sources. */
 void orc_rhs(const orc_params *P, const double *pr, double *rhs, double dx, double dy, double dz) {
     int nrhs = (P->physics == 1 ? 9 : 5) + P->n_passive;
-    memset(rhs, 0, sizeof(double) * (size_t)nrhs * NXF * NYF * NZF);
+    const size_t N = (size_t)NXF * NYF * NZF;
+    memset(rhs, 0, sizeof(double) * (size_t)nrhs * N);
+    const double *pu = pr;
+    double *tmp = NULL;
+    if (P->gr_orthonormal && (P->gr_metric == 1 || P->gr_metric == 2)) {
+        tmp = (double *)malloc(sizeof(double) * (size_t)P->nv * N);
+        memcpy(tmp, pr, sizeof(double) * (size_t)P->nv * N);
+        const int ng = P->ng;
+        for (int i = ng; i < NXF - ng; ++i)
+            for (int j = ng; j < NYF - ng; ++j)
+                for (int k = ng; k < NZF - ng; ++k) {
+                    double x = gr_x(P, i, dx), y = ((double)(j - ng) + 0.5) * dy, z = ((double)(k - ng) + 0.5) * dz;
+                    double ax = pymax(gr_ks(x, y, z, P->gr_mass, NULL), 1e-12);
+                    for (int v = 1; v <= 3; ++v) tmp[(size_t)v * N + IX(0, i, j, k)] = PR(v, i, j, k) / ax;
+                }
+        pu = tmp;
+    }
     if (P->physics == 1) {
         if (P->recon_id == 0)
-            rmhd_rhs_muscl(P, pr, rhs, dx, dy, dz);
+            rmhd_rhs_muscl(P, pu, rhs, dx, dy, dz);
         else
-            rmhd_rhs_ppm_weno(P, pr, rhs, dx, dy, dz);
+            rmhd_rhs_ppm_weno(P, pu, rhs, dx, dy, dz);
     } else {
         if (P->recon_id == 0)
-            srhd_rhs_muscl(P, pr, rhs, dx, dy, dz);
+            srhd_rhs_muscl(P, pu, rhs, dx, dy, dz);
         else
-            srhd_rhs_ppm_weno(P, pr, rhs, dx, dy, dz);
+            srhd_rhs_ppm_weno(P, pu, rhs, dx, dy, dz);
     }
+    if (P->gr_metric != 0) gr_sources(P, pr, rhs, dx, dy, dz);
+    free(tmp);
 }
 
 double orc_max_char_speed(const orc_params *P, const double *pr) { return srhd_max_char_speed(P, pr); }

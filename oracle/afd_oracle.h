@@ -38,6 +38,10 @@ typedef struct orc_params {
     int n_passive, passive_offset;
     int resistive;
     double resistivity;
+    /* GRHD/GRMHD wrappers (core/grhd_core.py, core/grmhd_core.py): 0 minkowski, 1 schwarzschild,
+       2 kerr-schild, 3 any other GR_METRIC (sources from the Kerr-Schild lapse, no rescale) */
+    int gr_metric, gr_orthonormal, gr_x0;
+    double gr_mass, gr_gamma; /* GR_MASS; the Gamma of the source's enthalpy (GRHD: config GAMMA) */
 } orc_params;
 
 /* pointwise (batched over n states, row-major n x nvar) */

@@ -49,6 +49,7 @@ def _install_numba_stub():
 _install_numba_stub()
 sys.path.insert(0, REF)
 from core import srhd_core, rmhd_core, boundary, nozzle, source_terms, chemistry, dissipation  # noqa: E402
+from core import grhd_core, grmhd_core  # noqa: E402
 
 
 def _save(name, **arrays):
@@ -533,11 +534,66 @@ def gen_dissipation():
 
 
 # ---------------------------------------------------------------------------
+# GRHD / GRMHD wrappers (core/grhd_core.py, core/grmhd_core.py, core/gr_metric.py)
+# ---------------------------------------------------------------------------
+GR_RHS_CASES = [
+    # physics, GR_METRIC, ORTHONORMAL_FLUX, GR_MASS, GAMMA, recon, limiter, riemann
+    ("grhd", "schwarzschild", True, 0.05, 1.4, "muscl", "mc", "hllc"),
+    ("grhd", "kerr-schild", False, 0.05, 5.0 / 3.0, "ppm", "mc", "hlle"),
+    ("grhd", "kerr-schild", True, 0.1, 5.0 / 3.0, "weno", "mc", "hllc"),
+    ("grhd", "kerr", True, 0.05, 5.0 / 3.0, "muscl", "vanleer", "hlle"),
+    ("grhd", "minkowski", True, 0.05, 5.0 / 3.0, "muscl", "mc", "hllc"),
+    ("grmhd", "schwarzschild", True, 0.05, 4.0 / 3.0, "weno", "mc", "hlld_full"),
+    ("grmhd", "kerr-schild", True, 0.05, 4.0 / 3.0, "muscl", "minmod", "hlld"),
+]
+GR_STEP_CASES = [  # index into GR_RHS_CASES, RK order
+    (0, 3), (5, 2),
+]
+
+
+def gr_configure(phys, metric, ortho, mass, gamma, recon, lim, riem):
+    gcfg = dict(GR_METRIC=metric, ORTHONORMAL_FLUX=ortho, GR_MASS=mass)
+    if phys == "grhd":
+        srhd_configure(RECON=recon, LIMITER=lim, RIEMANN=riem, GAMMA=gamma)
+        grhd_core.configure(gcfg)
+        return grhd_core
+    rmhd_configure(RECON=recon, LIMITER=lim, RIEMANN=riem, GAMMA=gamma)
+    grmhd_core.configure(gcfg)
+    return grmhd_core
+
+
+def gen_gr():
+    shape = (10, 11, 12)
+    ng = 3
+    dx, dy, dz = 1.0 / 4, 1.0 / 5, 1.0 / 6
+    arrs = {}
+    for c, case in enumerate(GR_RHS_CASES):
+        core = gr_configure(*case)
+        pr = (rmhd_state if case[0] == "grmhd" else srhd_state)(shape, 960 + c, jet=False)
+        arrs[f"rhs_in_{c}"] = pr
+        f = core.compute_rhs_grhd if case[0] == "grhd" else core.compute_rhs_grmhd
+        arrs[f"rhs_{c}"] = f(pr, dx, dy, dz, 0, ng)
+        print(f"[golden] gr rhs {case}", flush=True)
+    dt = 0.02
+    for c, (ci, rk) in enumerate(GR_STEP_CASES):
+        core = gr_configure(*GR_RHS_CASES[ci])
+        pr = arrs[f"rhs_in_{ci}"]
+        arrs[f"step_{c}"] = (core.step_ssprk3 if rk == 3 else core.step_ssprk2)(pr, dx, dy, dz, dt, 0, ng)
+        print(f"[golden] gr step {GR_RHS_CASES[ci]} rk{rk}", flush=True)
+    _save("gr", rhs_cases=np.array([repr(c) for c in GR_RHS_CASES]), step_cases=np.array(GR_STEP_CASES),
+          dxyz=np.array([dx, dy, dz]), dt=np.array(dt), ng=np.array(ng), **arrs)
+
+
+# ---------------------------------------------------------------------------
 # Driver loop (restates the loop order of solvers/srhd3d_mpi_muscl.py:276-326, one rank)
 # ---------------------------------------------------------------------------
 def run_driver(s, pr, nsteps, rng):
-    core = rmhd_core if s["PHYSICS"] == "rmhd" else srhd_core
+    phys = s["PHYSICS"]
+    core = rmhd_core if phys in ("rmhd", "grmhd") else srhd_core
     core.configure(s)
+    gr = {"grhd": grhd_core, "grmhd": grmhd_core}.get(phys)
+    if gr is not None:
+        gr.configure(s)
     ng = s["NG"]
     dx, dy, dz = s["Lx"] / s["NX"], s["Ly"] / s["NY"], s["Lz"] / s["NZ"]
     y0, z0 = s["JET_CENTER"][1], s["JET_CENTER"][2]
@@ -546,7 +602,7 @@ def run_driver(s, pr, nsteps, rng):
     nozzle.apply_nozzle_left_x(pr, dx, dy, dz, s["NY"], s["NZ"], y0, z0, rng, s)
     dts = []
     for step in range(nsteps):
-        if s["PHYSICS"] == "hydro":
+        if s["PHYSICS"] in ("hydro", "grhd"):
             amax = srhd_core.max_char_speed(pr, *pr.shape[1:])
         else:
             amax = 1.0
@@ -554,7 +610,10 @@ def run_driver(s, pr, nsteps, rng):
         boundary.apply_periodic_yz(pr, ng)
         boundary.apply_outflow_right_x(pr, ng)
         nozzle.apply_nozzle_left_x(pr, dx, dy, dz, s["NY"], s["NZ"], y0, z0, rng, s)
-        pr = (core.step_ssprk3 if s["RK_ORDER"] == 3 else core.step_ssprk2)(pr, dx, dy, dz, dt)
+        if gr is not None:
+            pr = (gr.step_ssprk3 if s["RK_ORDER"] == 3 else gr.step_ssprk2)(pr, dx, dy, dz, dt, 0, ng)
+        else:
+            pr = (core.step_ssprk3 if s["RK_ORDER"] == 3 else core.step_ssprk2)(pr, dx, dy, dz, dt)
         boundary.apply_periodic_yz(pr, ng)
         boundary.apply_outflow_right_x(pr, ng)
         nozzle.apply_nozzle_left_x(pr, dx, dy, dz, s["NY"], s["NZ"], y0, z0, rng, s)
@@ -581,7 +640,8 @@ def driver_settings(**kw):
 
 
 def init_block(s, noise_seed):
-    nv = (9 if s["PHYSICS"] == "rmhd" else (15 if s.get("DISSIPATION_ENABLED", False) else 5)) + s["N_PASSIVE"]
+    nv = (9 if s["PHYSICS"] in ("rmhd", "grmhd") else (15 if s.get("DISSIPATION_ENABLED", False) else 5)) + \
+        s["N_PASSIVE"]
     ng = s["NG"]
     pr = np.zeros((nv, s["NX"] + 2 * ng, s["NY"] + 2 * ng, s["NZ"] + 2 * ng))
     pr[0] = s["RHO_AMB"]; pr[4] = s["P_AMB"]
@@ -663,6 +723,20 @@ def gen_driver_dissipation():
     _save("driver_diss_10", pr0=pr0, out=out, dts=dts, settings=np.array(repr(s)))
 
 
+def gen_driver_gr():
+    # GRHD jet, MUSCL-MC + HLLC + SSPRK3, Schwarzschild metric (isotropic lapse sources, Kerr-Schild
+    # lapse in the flux velocities) with the hole at the domain corner, Gamma = 1.4 in the sources
+    s = driver_settings(NX=10, NY=9, NZ=8, NG=2, PHYSICS="grhd", GAMMA=1.4, RECON="muscl", LIMITER="mc",
+                        RIEMANN="hllc", RK_ORDER=3, NOZZLE_PERTURB="random", TURB_PAMP=0.05, SHEAR_THICK=0.02,
+                        JET_RADIUS=0.3, GAMMA_JET=2.0, GR_METRIC="schwarzschild", GR_MASS=0.05,
+                        ORTHONORMAL_FLUX=True)
+    pr0 = init_block(s, 6161)
+    t0 = time.time()
+    out, dts = run_driver(s, pr0.copy(), 3, np.random.default_rng(1234))
+    print(f"[golden] grhd driver {time.time()-t0:.1f}s: rho [{out[0].min():.3e}, {out[0].max():.3e}]", flush=True)
+    _save("driver_grhd_10", pr0=pr0, out=out, dts=dts, settings=np.array(repr(s)))
+
+
 def gen_driver_32():
     # The 32^3 SRHD jet: MUSCL-minmod + HLLE + SSPRK2, CFL 0.4, 20 steps, +-1% density noise
     # (default_rng(12345)), nozzle radius 4 cells, W_jet = 7.0888; reference periodic y/z boundaries.
@@ -676,7 +750,7 @@ def gen_driver_32():
 
 GENS = dict(pointwise=gen_pointwise, rhs=gen_rhs, steps=gen_steps, bc=gen_bc, sources=gen_sources,
             driver_small=gen_driver_small, driver_sources=gen_driver_sources,
-            dissipation=gen_dissipation, driver_diss=gen_driver_dissipation)
+            dissipation=gen_dissipation, driver_diss=gen_driver_dissipation, gr=gen_gr, driver_gr=gen_driver_gr)
 
 
 def main():

@@ -12,8 +12,12 @@ import oracle as O
 
 
 def params_from_settings(s, shape):
-    rm = s["PHYSICS"] == "rmhd"
+    rm = s["PHYSICS"] in ("rmhd", "grmhd")
+    gr = s["PHYSICS"] in ("grhd", "grmhd")
     return O.make_params("rmhd" if rm else "hydro", shape[1:], nv=shape[0], ng=s["NG"],
+                         gr_metric=s.get("GR_METRIC", "minkowski") if gr else "minkowski",
+                         gr_mass=s.get("GR_MASS", 1.0), orthonormal_flux=s.get("ORTHONORMAL_FLUX", True),
+                         gr_gamma=s.get("GAMMA", 5.0 / 3.0),
                          gamma=s["GAMMA"] if rm else None, p_max=s["P_MAX"], v_max=s["V_MAX"],
                          glm_ch=s.get("GLM_CH", 1.0), glm_cp=s.get("GLM_CP", 0.1), limiter=s["LIMITER"],
                          recon=s["RECON"], riemann=s["RIEMANN"], rk_order=s["RK_ORDER"],
@@ -66,7 +70,7 @@ def run(s, pr, nsteps, seed=1234, b_amb=(0.0, 0.0, 0.0), bc_y="periodic", bc_z="
     bcs(pr)
     dts = []
     for _ in range(nsteps):
-        amax = O.max_char_speed(P, pr) if s["PHYSICS"] == "hydro" else 1.0
+        amax = O.max_char_speed(P, pr) if s["PHYSICS"] in ("hydro", "grhd") else 1.0
         dt = s["CFL"] * min(dx, dy, dz) / max(amax, 1e-12)
         bcs(pr)
         pr = O.step(P, pr, dx, dy, dz, dt)

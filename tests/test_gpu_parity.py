@@ -153,7 +153,7 @@ def run_gpu_driver(s, pr0, nsteps, seed=1234):
         d.apply_bcs(rng)
         dts = []
         for _ in range(nsteps):
-            amax = d.max_char_speed() if s["PHYSICS"] == "hydro" else 1.0
+            amax = d.max_char_speed() if s["PHYSICS"] in ("hydro", "grhd") else 1.0
             dt = s["CFL"] * min(dx, dy, dz) / max(amax, 1e-12)
             d.apply_bcs(rng)
             d.step(dt)
@@ -165,7 +165,7 @@ def run_gpu_driver(s, pr0, nsteps, seed=1234):
         d.close()
 
 
-@pytest.mark.parametrize("name", ["driver_srhd_10", "driver_rmhd_8", "driver_sources_10", "driver_diss_10", "driver_srhd_32"])
+@pytest.mark.parametrize("name", ["driver_srhd_10", "driver_rmhd_8", "driver_sources_10", "driver_diss_10", "driver_grhd_10", "driver_srhd_32"])
 def test_driver_golden(name):
     path = os.path.join(G, name + ".npz")
     if not os.path.exists(path):

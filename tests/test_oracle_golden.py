@@ -165,7 +165,7 @@ def test_bc_nozzle_two_temperature():
 
 
 # ---------------------------------------------------------------- driver loops
-@pytest.mark.parametrize("name", ["driver_srhd_10", "driver_rmhd_8", "driver_sources_10", "driver_diss_10", "driver_srhd_32"])
+@pytest.mark.parametrize("name", ["driver_srhd_10", "driver_rmhd_8", "driver_sources_10", "driver_diss_10", "driver_grhd_10", "driver_srhd_32"])
 def test_driver_loop(name):
     path = os.path.join(G, name + ".npz")
     if not os.path.exists(path):
@@ -338,3 +338,60 @@ def test_dissipative_driver_conditioning():
     err = max(float(np.abs(out2[v] - out[v]).max() / np.abs(out[v]).max()) for v in range(out.shape[0]))
     l1 = max(float(np.abs(out2[v] - out[v]).sum() / np.abs(out[v]).sum()) for v in range(out.shape[0]))
     assert 1e-8 < err < 1e-5 and l1 < 2e-6
+
+
+# ---------------------------------------------------------------- GRHD / GRMHD wrappers
+def gr_params(case, shape, nv, ng, rk=2):
+    phys, metric, ortho, mass, gamma, recon, lim, riem = case
+    rm = phys == "grmhd"
+    return O.make_params("grmhd" if rm else "hydro", shape, nv=nv, ng=ng, gamma=gamma if rm else None,
+                         p_max=2.0, v_max=0.9999, limiter=lim, recon=recon, riemann=riem, rk_order=rk,
+                         gr_metric=metric, gr_mass=mass, orthonormal_flux=ortho, gr_gamma=gamma)
+
+
+def test_gr_golden():
+    """compute_rhs_grhd / compute_rhs_grmhd (velocity rescale by the Kerr-Schild lapse with
+    ORTHONORMAL_FLUX, lapse-gradient sources for every metric but Minkowski, an unknown metric name
+    taking the Kerr-Schild sources only) and the GR SSPRK steps, bitwise."""
+    f = load("gr")
+    dx, dy, dz = f["dxyz"]
+    ng = int(f["ng"])
+    cases = [ast.literal_eval(str(c)) for c in f["rhs_cases"]]
+    for c, case in enumerate(cases):
+        pr = f[f"rhs_in_{c}"]
+        P = gr_params(case, pr.shape[1:], pr.shape[0], ng)
+        assert_bitwise(O.rhs(P, pr, dx, dy, dz), f[f"rhs_{c}"], f"gr rhs {case}")
+    for c, (ci, rk) in enumerate(f["step_cases"]):
+        pr = f[f"rhs_in_{ci}"]
+        P = gr_params(cases[ci], pr.shape[1:], pr.shape[0], ng, rk=int(rk))
+        assert_bitwise(O.step(P, pr, dx, dy, dz, float(f["dt"])), f[f"step_{c}"], f"gr step {cases[ci]}")
+
+
+def test_gr_oracle_properties():
+    """Closed forms of the wrappers: with Minkowski the RHS equals the flat one; a uniform state at
+    rest under a Kerr-Schild metric without the rescale has a zero flux divergence, so the RHS is the
+    source alone: rhs[1..3] = (rho h + p) d ln(alpha)/dx^i with alpha = (1 + 2M/r)^-1/2, rhs[4] = 0."""
+    shape = (10, 10, 10)
+    pr = np.zeros((5,) + shape)
+    pr[0], pr[4] = 0.8, 0.05
+    M, G = 0.1, 1.4
+    P = O.make_params("hydro", shape, ng=3, p_max=2.0, v_max=0.9999, gr_metric="kerr-schild", gr_mass=M,
+                      orthonormal_flux=False, gr_gamma=G)
+    r = O.rhs(P, pr, 0.2, 0.2, 0.2)
+    x = (np.arange(10) - 3 + 0.5) * 0.2
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    R = np.sqrt(X ** 2 + Y ** 2 + Z ** 2) + 1e-12
+    H = M / R
+    dlna = H / (R * (1 + 2 * H))  # d ln (1 + 2M/r)^-1/2 / dr
+    w = 0.8 * (1 + G / (G - 1) * 0.05 / 0.8) + 0.05
+    core = (slice(3, -3),) * 3
+    for m, Xm in ((1, X), (2, Y), (3, Z)):
+        assert np.allclose(r[m][core], (w * dlna * Xm / R)[core], rtol=1e-12, atol=0)
+    assert np.abs(r[4][core]).max() == 0.0 and np.abs(r[0]).max() == 0.0
+    # Minkowski: the flat RHS
+    st = np.random.default_rng(4)
+    pr = np.stack([1 + 0.1 * st.random(shape), 0.1 * st.random(shape), 0.1 * st.random(shape),
+                   0.1 * st.random(shape), 0.01 + 0.01 * st.random(shape)])
+    flat = O.make_params("hydro", shape, ng=3, p_max=2.0, v_max=0.9999)
+    mink = O.make_params("hydro", shape, ng=3, p_max=2.0, v_max=0.9999, gr_metric="minkowski")
+    assert np.array_equal(O.rhs(flat, pr, 0.2, 0.2, 0.2), O.rhs(mink, pr, 0.2, 0.2, 0.2))
