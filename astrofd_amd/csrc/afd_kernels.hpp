@@ -598,8 +598,12 @@ AFD_D bool cell_c2p(const UpdArgs<F> &a, const double *U, bool live, long c, dou
 // Stage update of one cell: flux divergence + sources, SSPRK combination into Uc (FP64), the
 // passives on primitives, and the con2prim of the new state into Pout.  LEARNED selects the
 // network's Newton start; a separate instantiation keeps its registers out of the plain kernel.
+// The learned instantiation is held to 3 waves per SIMD (<= 168 VGPRs, no spills): left alone
+// the compiler takes 190 registers (2 waves), and on the 256^3 RMHD WENO5 + HLLD step the
+// kernel ran 16 % slower that way (4 waves spills and is 6 % slower than 3).
 template <int NB, typename F, bool LEARNED>
-__global__ __launch_bounds__(BLOCK) void k_update(UpdArgs<F> a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LEARNED ? 3 : 1))) void k_update(
+    UpdArgs<F> a) {
     const Grid &g = a.g;
     const long c = (long)blockIdx.x * BLOCK + threadIdx.x;
     const bool live = c < g.ncell;

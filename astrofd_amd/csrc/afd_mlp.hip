@@ -19,6 +19,12 @@ int mlp_alloc(MlpDevice &m, const std::function<int(void **, size_t)> &alloc) {
 }
 
 // Master FP32 weights -> BF16 A-operand blocks (lane order, afd_mlp_device.hpp) + FP32 biases.
+// A bias b as hi + lo: hi = the nearest BF16 of b, lo = the nearest BF16 of b - hi (exact in FP32).
+AFD_D float bias_part(float b, int which) {
+    const float hi = (float)(__bf16)b;
+    return which == 0 ? hi : b - hi;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_mlp_pack(const float *master, void *wout) {
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     __bf16 *blk = (__bf16 *)wout;
@@ -27,9 +33,13 @@ __global__ __launch_bounds__(BLOCK) void k_mlp_pack(const float *master, void *w
         const int e = t % 8, lane = (t / 8) % 64, b = t / 512;
         const int m = lane % 32, k = 8 * (lane / 32) + e;
         float v = 0.0f;
-        if (b < BLK_W2) {
+        if (b >= BLK_B2) {  // b2 as the split pair hi + lo in K columns 0, 1
+            const float bb = master[MLP_OFF_B2 + 32 * (b - BLK_B2) + m];
+            if (k < 2) v = bias_part(bb, k);
+        } else if (b < BLK_W2) {
             int mt = b - BLK_W1;
             if (k < MLP_IN) v = master[MLP_OFF_W1 + (32 * mt + m) * MLP_IN + k];
+            else if (k < MLP_IN + 2) v = bias_part(master[MLP_OFF_B1 + 32 * mt + m], k - MLP_IN);
         } else if (b < BLK_W3) {
             int idx = b - BLK_W2, mt = idx / 4, kb = idx % 4;
             v = master[MLP_OFF_W2 + (32 * mt + m) * 64 + 16 * kb + k];

@@ -10,6 +10,10 @@
 //   D (32 x 32):  lane l holds D[8 (i / 4) + 4 (l / 32) + i % 4][l % 32],  i = 0..15
 // A D-layout result becomes the next layer's B operand after one exchange of four values
 // between lanes l and l ^ 32 per 16-row K block.  Accumulation is FP32.
+// The biases b1 and b2 ride in the MFMAs as a split pair b = hi + lo of BF16 values (hi the
+// nearest BF16 of b, lo that of the remainder: 16 significant bits) against constant-one input
+// rows: b1 in columns 5, 6 of W1 (inputs 5, 6 are one), b2 in an extra K block whose rows 0, 1
+// are one.  No bias lives in registers across the layers.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -22,12 +26,13 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 // Packed device weights: NBLK blocks of 64 lanes x 8 BF16 (A operands), then FP32 biases.
 enum MlpBlock {
-    BLK_W1 = 0,    // W1 [64 x 16]: m-tile 0..1, k-block 0 (inputs 5..15 zero)
+    BLK_W1 = 0,    // [W1 b1_hi b1_lo 0 ...] [64 x 16]: m-tile 0..1, k-block 0 (inputs 5, 6 are one)
     BLK_W2 = 2,    // W2 [64 x 64]: m-tile 0..1 x k-block 0..3  (index 2 + 4 mt + kb)
     BLK_W3 = 10,   // W3 [32 x 64]: k-block 0..3 (rows 2..31 zero)
     BLK_W3T = 14,  // W3^T [64 x 16]: m-tile 0..1 (columns 2..15 zero)
     BLK_W2T = 16,  // W2^T [64 x 64]: m-tile 0..1 x k-block 0..3
-    MLP_NBLK = 24
+    BLK_B2 = 24,   // [b2_hi b2_lo 0 ...] [64 x 16]: m-tile 0..1, against the ones block
+    MLP_NBLK = 26
 };
 constexpr int MLP_BIAS_FLOATS = 64 + 64 + 2;
 
@@ -99,26 +104,23 @@ struct MlpTile {
     bf16x8 h1b[4], h2b[4];
 };
 
-// The biases enter as the MFMA's initial accumulator: lane l's element i is row drow(i, l).
-AFD_D floatx16 bias16(const float *b, int lane) {
-    floatx16 r;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) r[i] = b[drow(i, lane)];
-    return r;
+// B operand of the ones block: K rows 0 and 1 are one for every sample column.
+AFD_D bf16x8 ones_operand(bool upper) {
+    return upper ? pack8(0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f) : pack8(1.f, 1.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f);
 }
 
 AFD_D void mlp_tile_forward(const MlpDevW &w, int lane, bf16x8 xb, MlpTile &t) {
     floatx16 h[2];
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt) {
-        t.z1[mt] = mfma_bf16(w.blk[(BLK_W1 + mt) * 64 + lane], xb, bias16(w.bias + 32 * mt, lane));
+        t.z1[mt] = mfma_bf16(w.blk[(BLK_W1 + mt) * 64 + lane], xb, zero16());
 #pragma unroll
         for (int i = 0; i < 16; ++i) h[mt][i] = silu(t.z1[mt][i]);
     }
     d_to_b(h, lane, t.h1b);
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt) {
-        floatx16 acc = bias16(w.bias + 64 + 32 * mt, lane);
+        floatx16 acc = mfma_bf16(w.blk[(BLK_B2 + mt) * 64 + lane], ones_operand(lane >= 32), zero16());
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb) acc = mfma_bf16(w.blk[(BLK_W2 + 4 * mt + kb) * 64 + lane], t.h1b[kb], acc);
 #pragma unroll
@@ -134,8 +136,9 @@ AFD_D void mlp_tile_forward(const MlpDevW &w, int lane, bf16x8 xb, MlpTile &t) {
     t.o = o;
 }
 
+// B operand of the inputs: the five features and the two ones of the b1 columns.
 AFD_D bf16x8 input_operand(const float x[5], bool upper) {
-    return upper ? pack8(0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f) : pack8(x[0], x[1], x[2], x[3], x[4], 0.f, 0.f, 0.f);
+    return upper ? pack8(0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f) : pack8(x[0], x[1], x[2], x[3], x[4], 1.f, 1.f, 0.f);
 }
 
 // Whole-wave inference: lane l supplies the features of its sample, gets its two outputs.

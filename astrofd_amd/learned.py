@@ -48,13 +48,17 @@ def _unit_vectors(rng, n):
     return v / np.linalg.norm(v, axis=1, keepdims=True)
 
 
-def sample_primitives(n, seed, gamma=GAMMA):
+def sample_primitives(n, seed, gamma=GAMMA, slow_frac=0.5):
     """Primitive RMHD states (n x 9): rho log-uniform [1e-4, 1e2], W uniform [1, 50],
     p/rho log-uniform [1e-4, 1e2], magnetisation sigma = b^2/(rho h) log-uniform [1e-3, 10],
-    isotropic directions of v and B, psi = 0."""
+    isotropic directions of v and B, psi = 0.  A fraction `slow_frac` of the states instead
+    draws W - 1 log-uniform in [1e-11, 49] (|v| from ~4e-6 up): a uniform W leaves the nearly
+    static gas that fills most of a jet's box (|v| < 0.02 has W - 1 < 2e-4) out of the data."""
     rng = np.random.default_rng(seed)
     rho = 10.0 ** rng.uniform(-4.0, 2.0, n)
     W = rng.uniform(1.0, 50.0, n)
+    slow = rng.uniform(0.0, 1.0, n) < slow_frac
+    W[slow] = 1.0 + 10.0 ** rng.uniform(-11.0, np.log10(49.0), int(slow.sum()))
     p = rho * 10.0 ** rng.uniform(-4.0, 2.0, n)
     sigma = 10.0 ** rng.uniform(-3.0, 1.0, n)
     vhat = _unit_vectors(rng, n)
@@ -87,12 +91,12 @@ def make_context(device=0, learned=False):
     return Domain(p)
 
 
-def build_dataset(dom, n, seed, chunk=1 << 22):
+def build_dataset(dom, n, seed, chunk=1 << 22, slow_frac=0.5):
     """(X, Y) of valid pairs from n sampled primitives, and the valid fraction."""
     Xs, Ys, nvalid = [], [], 0
     for off in range(0, n, chunk):
         m = min(chunk, n - off)
-        q = sample_primitives(m, seed + off // chunk)
+        q = sample_primitives(m, seed + off // chunk, slow_frac=slow_frac)
         X, Y, ok = dom.mlp_dataset_from_prims(q)
         Xs.append(X[ok])
         Ys.append(Y[ok])
@@ -125,7 +129,7 @@ def evaluate(w, n=1 << 20, seed=99, device=0, return_states=False):
     changes, so the learned path can only gain there.  Iterations are the solver's own counts:
     hydro fixed-point iterations + Newton f(Z) evaluations (plain), Newton evaluations from the
     network's guess (+ the plain solve when the judge rejects)."""
-    q = sample_primitives(n, seed)
+    q = sample_primitives(n, seed, slow_frac=0.0)  # the cfg3 distribution as specified
     with make_context(device) as plain, make_context(device, learned=True) as lrn:
         lrn.set_mlp_weights(w)
         _, _, newton_valid = plain.mlp_dataset_from_prims(q)
@@ -192,6 +196,7 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--eval-states", type=int, default=1 << 20)
+    ap.add_argument("--slow-frac", type=float, default=0.5, help="fraction of states with log-uniform W - 1")
     a = ap.parse_args(argv)
 
     def log(msg):
@@ -199,7 +204,7 @@ def main(argv=None):
 
     t0 = time.perf_counter()
     with make_context() as dom:
-        X, Y, frac = build_dataset(dom, a.samples, a.seed)
+        X, Y, frac = build_dataset(dom, a.samples, a.seed, slow_frac=a.slow_frac)
         log(f"dataset: {X.shape[0]} valid pairs of {a.samples} ({frac:.4%}) in {time.perf_counter() - t0:.1f} s")
         t1 = time.perf_counter()
         w, hist = train(dom, X, Y, epochs=a.epochs, batch=a.batch, lr0=a.lr, log=log)
@@ -207,7 +212,7 @@ def main(argv=None):
     log(f"training: {hist.size} Adam steps in {t_train:.1f} s")
     ev = evaluate(w, n=a.eval_states)
     log(json.dumps(ev))
-    meta = dict(samples=a.samples, valid_fraction=frac, epochs=a.epochs, batch=a.batch, lr=a.lr,
+    meta = dict(samples=a.samples, slow_frac=a.slow_frac, valid_fraction=frac, epochs=a.epochs, batch=a.batch, lr=a.lr,
                 seed=a.seed, steps=int(hist.size), final_loss=float(hist[-256:].mean()),
                 train_seconds=t_train, eval=ev)
     save(a.out, w, meta)

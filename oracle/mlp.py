@@ -10,9 +10,11 @@ W3[2][64], b3[2]  (4674 values).  Loss = mean over samples of the summed
 squared error of the two outputs.
 
 `emulate=True` reproduces where the GPU kernels round operands to BF16
-(weights, inputs, hidden activations, back-propagated deltas; accumulation and
-biases stay FP32, here FP64), so that the GPU result differs from the oracle
-only by FP32 accumulation order and the hardware exp/reciprocal.
+(weights, inputs, hidden activations, back-propagated deltas; accumulation is
+FP32, here FP64; the biases b1 and b2 enter as the pair hi + lo of BF16 values,
+b1 and b2 of 16 significant bits, and b3 in FP32), so that the GPU result
+differs from the oracle only by FP32 accumulation order and the hardware
+exp/reciprocal.
 `emulate=False` is the plain FP64 network, pinned in the tests by central
 finite differences of the loss and by torch autograd on the CPU.
 
@@ -61,6 +63,13 @@ def _ident(a):
     return np.asarray(a, dtype=np.float64)
 
 
+def bias_split(b):
+    """hi + lo with hi = bf16(b) and lo = bf16(b - hi): the biases as the GPU's MFMAs see them."""
+    b = np.asarray(b, dtype=np.float32).astype(np.float64)
+    hi = bf16(b)
+    return hi + bf16(b - hi)
+
+
 def sigmoid(z):
     return 0.5 * (1.0 + np.tanh(0.5 * z))
 
@@ -77,11 +86,12 @@ def dsilu(z):
 def forward(w, X, emulate=True):
     """Outputs (n x 2) and the intermediates of the forward pass."""
     r = bf16 if emulate else _ident
+    rb = bias_split if emulate else _ident
     W1, b1, W2, b2, W3, b3 = unpack(w)
     xb = r(X)                                  # n x 5
-    z1 = xb @ r(W1).T + b1                     # n x 64
+    z1 = xb @ r(W1).T + rb(b1)                 # n x 64
     h1 = r(silu(z1))
-    z2 = h1 @ r(W2).T + b2
+    z2 = h1 @ r(W2).T + rb(b2)
     h2 = r(silu(z2))
     o = h2 @ r(W3).T + b3
     return o, dict(xb=xb, z1=z1, h1=h1, z2=z2, h2=h2)
