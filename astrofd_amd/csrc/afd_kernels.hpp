@@ -95,6 +95,24 @@ template <int DIR> AFD_D long dstride(const Grid &g) { return DIR == 0 ? g.sx : 
 
 // Decompose a linear thread index over the face box of direction DIR
 // (faces [r0, r1] along DIR, cells [r0, r1) across it), z fastest.
+// Workgroup b of a launch runs on XCD b % 8 (the round-robin dispatch of the 8-XCD MI355X).
+// This maps b to a logical block so that each XCD walks one contiguous eighth of the grid in
+// order: rows j and j +- 1 of a plane then share an XCD and its L2.  A bijection on [0, nb).
+AFD_D long xcd_chunked_block(long b, long nb) {
+    const long q = nb / 8, r = nb % 8, x = b % 8, idx = b / 8;
+    return x < r ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
+}
+
+// Thread index of a face kernel.  The y faces' stencil spans rows j-3 .. j+2 of one plane: with
+// the plain order those rows land on six XCDs and every XCD fetches them from HBM (3x the
+// compulsory bytes measured at 256^3), so the y kernels walk XCD-contiguous chunks.  The x
+// faces keep the plain order, under which planes i-3 .. i+2 of a row share an XCD (j % 8) at a
+// reuse distance of one plane / 8 rows.
+template <int DIR> AFD_D long face_thread() {
+    const long b = DIR == 1 ? xcd_chunked_block(blockIdx.x, gridDim.x) : (long)blockIdx.x;
+    return b * BLOCK + threadIdx.x;
+}
+
 template <int DIR> AFD_D bool face_coords(const Grid &g, long t, int &i, int &j, int &k) {
     int n[3];
 #pragma unroll
@@ -142,7 +160,7 @@ template <int RECON, int RIEM, int DIR, typename T>
 __global__ __launch_bounds__(BLOCK) void k_face_srhd(FaceArgs<T> a) {
     const Grid &g = a.g;
     int i, j, k;
-    if (!face_coords<DIR>(g, (long)blockIdx.x * BLOCK + threadIdx.x, i, j, k)) return;
+    if (!face_coords<DIR>(g, face_thread<DIR>(), i, j, k)) return;
     const long c = cidx(g, i, j, k), s = dstride<DIR>(g);
     double qLd[5], qRd[5];
 #pragma unroll
@@ -261,7 +279,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(FaceOcc<T
 void k_face_rmhd(FaceArgs<T> a) {
     const Grid &g = a.g;
     int i, j, k;
-    if (!face_coords<DIR>(g, (long)blockIdx.x * BLOCK + threadIdx.x, i, j, k)) return;
+    if (!face_coords<DIR>(g, face_thread<DIR>(), i, j, k)) return;
     const long c = cidx(g, i, j, k);
     RmhdFace<DIR, T> f;
     rmhd_face_setup<RECON, DIR, T>(a, c, f);
