@@ -103,13 +103,14 @@ AFD_D long xcd_chunked_block(long b, long nb) {
     return x < r ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
 }
 
-// Thread index of a face kernel.  The y faces' stencil spans rows j-3 .. j+2 of one plane: with
-// the plain order those rows land on six XCDs and every XCD fetches them from HBM (3x the
-// compulsory bytes measured at 256^3), so the y kernels walk XCD-contiguous chunks.  The x
-// faces keep the plain order, under which planes i-3 .. i+2 of a row share an XCD (j % 8) at a
-// reuse distance of one plane / 8 rows.
+// Thread index of a face kernel.  The x and y faces' stencils span six rows (k lines) of the
+// grid along the face normal.  Faces are ordered k fastest, then along the normal, so that the
+// rows a stencil shares are consecutive blocks, and the blocks walk XCD-contiguous chunks so
+// that those rows share an XCD and its L2.  (With the plain order the y stencil's rows landed on
+// six XCDs and the y kernels fetched 3x their compulsory bytes from HBM at 256^3.)  The z
+// stencil lies within a row.
 template <int DIR> AFD_D long face_thread() {
-    const long b = DIR == 1 ? xcd_chunked_block(blockIdx.x, gridDim.x) : (long)blockIdx.x;
+    const long b = DIR != 2 ? xcd_chunked_block(blockIdx.x, gridDim.x) : (long)blockIdx.x;
     return b * BLOCK + threadIdx.x;
 }
 
@@ -121,8 +122,13 @@ template <int DIR> AFD_D bool face_coords(const Grid &g, long t, int &i, int &j,
     if (t >= tot) return false;
     k = (int)(t % n[2]);
     long r = t / n[2];
-    j = (int)(r % n[1]);
-    i = (int)(r / n[1]);
+    if (DIR == 0) {  // k, then i (the normal), then j
+        i = (int)(r % n[0]);
+        j = (int)(r / n[0]);
+    } else {         // k, then j, then i
+        j = (int)(r % n[1]);
+        i = (int)(r / n[1]);
+    }
     i += g.r0[0];
     j += g.r0[1];
     k += g.r0[2];

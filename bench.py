@@ -196,7 +196,7 @@ def kernel_name(s, which, mixed, learned):
     T = "float" if mixed else "double"
     nb = 9 if s["PHYSICS"] == "rmhd" else 5
     if which == "stage_update_c2p":
-        return f"k_update<{nb}, {T}, {'true' if learned else 'false'}, true>"
+        return f"k_update<{nb}, {T}, {'true' if learned else 'false'}>"
     from astrofd_amd.domain import recon_id, riemann_id
     d = {"face_x": 0, "face_y": 1, "face_z": 2}[which]
     kind = "k_face_rmhd" if nb == 9 else "k_face_srhd"
@@ -242,7 +242,7 @@ def kernel_roofline(s, kern, mixed, learned=False, workload=None):
             "avg_ms": round(ms, 4), "bytes_per_launch": int(nbytes), "rocprof_name": kname}
 
 
-def cpu_baseline(s, n=64, steps=2, n1=32):
+def cpu_baseline(s, n=128, steps=2, n1=48):
     """The CPU oracle (C restatement of the reference, FP64, OpenMP) on a bounded sample of the same
     workload: `steps` steps (fills + SSPRK step + fills) at n^3 on all host cores, and one step at
     n1^3 on one core (the reference itself is serial per rank)."""
