@@ -76,22 +76,26 @@ AFD_D bf16x8 pack8(float a0, float a1, float a2, float a3, float a4, float a5, f
 }
 
 // D layout of a 64-row activation (two 32-row m-tiles) -> B operands of its four 16-row K blocks.
+// Lanes < 32 keep their `lo` rows and take the upper lanes' `lo`; lanes >= 32 take the lower
+// lanes' `hi` and keep their `hi`: one v_permlane32_swap per value pair (it exchanges the upper
+// half of its first operand with the lower half of its second), after which every lane packs
+// (first, second) alike.
 AFD_D void d_to_b(const floatx16 (&d)[2], int lane, bf16x8 (&bop)[4]) {
-    const bool upper = lane >= 32;
+    (void)lane;
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt) {
 #pragma unroll
         for (int kk = 0; kk < 2; ++kk) {
-            float lo[4], hi[4], rv[4];
+            float a[4], b[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                lo[r] = d[mt][8 * kk + r];      // rows 16kk + 4 (l/32) + r
-                hi[r] = d[mt][8 * kk + 4 + r];  // rows 16kk + 8 + 4 (l/32) + r
+                // lo: rows 16kk + 4 (l/32) + r; hi: rows 16kk + 8 + 4 (l/32) + r
+                const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(d[mt][8 * kk + r]),
+                                                                  __float_as_uint(d[mt][8 * kk + 4 + r]), false, false);
+                a[r] = __uint_as_float(sw[0]);
+                b[r] = __uint_as_float(sw[1]);
             }
-#pragma unroll
-            for (int r = 0; r < 4; ++r) rv[r] = __shfl_xor(upper ? lo[r] : hi[r], 32);
-            bop[2 * mt + kk] = upper ? pack8(rv[0], rv[1], rv[2], rv[3], hi[0], hi[1], hi[2], hi[3])
-                                     : pack8(lo[0], lo[1], lo[2], lo[3], rv[0], rv[1], rv[2], rv[3]);
+            bop[2 * mt + kk] = pack8(a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]);
         }
     }
 }
