@@ -66,7 +66,7 @@ struct afd_ctx {
     std::vector<hipEvent_t> ev_pool;
     // learned guess
     MlpDevice mlp{};
-    int *c2p_queue = nullptr;         // cells whose learned start the judge rejected (QUEUE_SLOTS regions)
+    int *c2p_queue = nullptr;         // cells whose learned start failed the acceptance test (QUEUE_SLOTS regions)
     unsigned *c2p_qcount = nullptr;
     long c2p_qcap = 0;
     std::vector<void *> allocs;

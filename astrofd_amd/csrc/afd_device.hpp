@@ -713,9 +713,9 @@ AFD_D void rmhd_c2p(const RmhdC2PParams &c, const double *U, double *q, C2PStats
     rmhd_recover<FAST>(c, Z, U, B2, SB, q0, q, st);
 }
 
-// Learned-guess con2prim: FD-Newton (the reference's iteration) from the network's Z0, judged by a
+// Learned-guess con2prim: FD-Newton (the reference's iteration) from the network's Z0, accepted by a
 // residual |f(Z)| < 1e-12 max(1, tau), a hundred times tighter than the reference's stopping test, and
-// by a valid recovery.  Returns false when the judge rejects; the caller then runs rmhd_c2p.
+// by a valid recovery.  Returns false when the acceptance test fails; the caller then runs rmhd_c2p.
 template <bool FAST = false>
 AFD_D bool rmhd_c2p_from_guess(const RmhdC2PParams &c, const double *U, double Z0, double *q, C2PStats &st) {
     if (!(Z0 > SMALL) || !isfinite(Z0)) return false;

@@ -388,7 +388,7 @@ template <typename F> struct UpdArgs {
     double gamma, gfac, p_max, v_max;
     const void *mlp;              // learned-guess weights (null: plain con2prim)
     int guess;                    // 0 plain, 1 learned, 2 previous-state Newton start
-    int *queue;                   // learned guess: cells the judge rejected, QUEUE_SLOTS regions
+    int *queue;                   // learned guess: cells whose start failed the acceptance test, QUEUE_SLOTS regions
     unsigned *qcount;             //   of qcap entries each, filled per workgroup slot (or null)
     long qcap;
     int ng, gr, gr_x0;            // GR sources: 0 none, 1 isotropic-Schwarzschild lapse, 2 Kerr-Schild
@@ -583,13 +583,13 @@ AFD_D double prim_Z(double gfac, double rho, double vx, double vy, double vz, do
 
 // con2prim of one cell's conserved state U (FP64).  LEARNED: the RMHD Newton solve starts from
 // the network's guess; guess == 2: from Z of the stage input's primitives (P0 at stage 0, else
-// Pin).  Either start is judged and backed by the plain solve: inline, or (queue set) by queueing
+// Pin).  Either start must pass the acceptance test and is backed by the plain solve: inline, or (queue set) by queueing
 // the rejected cell for k_c2p_queue.  Reached by every lane of the wave (the network runs
 // wave-wide); `live` marks lanes with a cell.  Returns whether q holds the cell's primitives.
 template <int NB, bool LEARNED, typename F>
 AFD_D bool cell_c2p(const UpdArgs<F> &a, const double *U, bool live, long c, double *q, C2PStats &st) {
     // The mixed RMHD con2prim uses the fast FP64 rcp/rsq: its hydro stage only seeds a Newton solve
-    // judged by its residual.  The SRHD con2prim's output is the fixed point's own branch (converged
+    // accepted by its residual.  The SRHD con2prim's output is the fixed point's own branch (converged
     // or the energy fallback), which ulp-level changes can flip, so it stays IEEE.
     constexpr bool FAST = sizeof(F) == 4 && NB == 9;
     if constexpr (NB == 9) {

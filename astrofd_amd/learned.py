@@ -3,7 +3,7 @@
 The network (5 -> 64 -> 64 -> 2, SiLU, BF16 MFMA kernels with FP32 accumulation in
 libafd) maps the scaled conserved state of a cell to (log10(p/D)/4, log10 W).  The stage
 update turns that into the Newton start Z0 = D W + Gamma/(Gamma-1) p W^2 and runs the
-reference's FD-Newton iteration (core/rmhd_core.py:273-290) from it; the judge accepts the
+reference's FD-Newton iteration (core/rmhd_core.py:273-290) from it; the acceptance test passes the
 result only when |f(Z)| < 1e-12 max(1, tau) within 12 evaluations and the recovery
 (core/rmhd_core.py:320-341) is valid, otherwise the plain con2prim runs.
 
@@ -128,7 +128,7 @@ def evaluate(w, n=1 << 20, seed=99, device=0, return_states=False):
     state; elsewhere it returns the bisection result or the hydro guess, which no initial guess
     changes, so the learned path can only gain there.  Iterations are the solver's own counts:
     hydro fixed-point iterations + Newton f(Z) evaluations (plain), Newton evaluations from the
-    network's guess (+ the plain solve when the judge rejects)."""
+    network's guess (+ the plain solve when the acceptance test fails)."""
     q = sample_primitives(n, seed, slow_frac=0.0)  # the cfg3 distribution as specified
     with make_context(device) as plain, make_context(device, learned=True) as lrn:
         lrn.set_mlp_weights(w)
@@ -158,8 +158,8 @@ def evaluate(w, n=1 << 20, seed=99, device=0, return_states=False):
         "plain_newton_evals_per_state": sp["newton_evals"] / n,
         "learned_hydro_iters_per_state": sl["hydro_iters"] / n,
         "learned_newton_evals_per_state": sl["newton_evals"] / n,
-        "judge_accept_rate": sl["learned_ok"] / n,
-        "judge_accept_rate_newton_valid": sl["learned_ok"] / max(nv, 1),
+        "accept_rate": sl["learned_ok"] / n,
+        "accept_rate_newton_valid": sl["learned_ok"] / max(nv, 1),
         "plain_bisections": sp["bisections"], "learned_bisections": sl["bisections"],
         "plain_invalid": sp["invalid"], "learned_invalid": sl["invalid"],
         "identical_outside_newton_valid": float(same.mean()) if same.size else 1.0,

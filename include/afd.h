@@ -40,8 +40,8 @@ extern "C" {
 #define AFD_PREC_FP64 0  /* everything in FP64 */
 #define AFD_PREC_MIXED 1 /* reconstruction + Riemann in FP32; Lorentz factor, con2prim, SSPRK in FP64 */
 #define AFD_C2P_PLAIN 0
-#define AFD_C2P_LEARNED 1 /* MLP initial guess for the RMHD Newton solve; the plain solve judges and falls back */
-#define AFD_C2P_PREVIOUS 2 /* Newton start from the stage input's primitives (Z = rho h W^2); same judge and fallback */
+#define AFD_C2P_LEARNED 1 /* MLP initial guess for the RMHD Newton solve; the plain solve checks it and falls back */
+#define AFD_C2P_PREVIOUS 2 /* Newton start from the stage input's primitives (Z = rho h W^2); same acceptance test and fallback */
 #define AFD_BC_PERIODIC 0
 #define AFD_BC_OUTFLOW 1
 
@@ -106,7 +106,7 @@ typedef struct afd_stats {
     uint64_t hydro_fallbacks; /* cells where the fixed point took the energy fallback */
     uint64_t bisections;      /* RMHD cells that reached bisection */
     uint64_t invalid;         /* RMHD cells whose recovery was invalid (hydro guess kept) */
-    uint64_t learned_ok;      /* RMHD cells solved from the learned guess (judge passed) */
+    uint64_t learned_ok;      /* RMHD cells solved from the learned guess (acceptance test passed) */
     uint64_t c2p_cells;       /* cells converted */
     double ms_face[3];        /* summed HIP-event time of the x, y, z face kernels (timing on) */
     double ms_update;         /* summed time of the stage-update (+ con2prim) kernel */

@@ -1,7 +1,8 @@
 """ctypes binding of the CPU FP64 oracle (oracle/afd_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product path
+cpu_baseline leg of bench.py, as the reference the GPU results are compared with.
+The product path
 (astrofd_amd) never imports this package.
 
 Arrays follow the reference layout: float64, C-order (nv, nx, ny, nz) with ghost

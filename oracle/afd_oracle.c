@@ -1,7 +1,8 @@
 /*
  * afd_oracle.c -- CPU FP64 restatement of AstroFD's SRHD/RMHD finite-volume step.
  *
- * TEST INFRASTRUCTURE ONLY (the checker): loaded by tests/, __graft_entry__.smoke()
+ * TEST INFRASTRUCTURE ONLY (the results of the GPU library are compared with it): loaded
+ * by tests/, __graft_entry__.smoke()
  * and bench.py's cpu_baseline leg; the product path never links or calls it.
  *
  * Every function follows the reference file:line it cites, in the reference's

@@ -1,7 +1,7 @@
 """GPU tests of the learned con2prim guess: the BF16-MFMA forward and backward kernels against
 the CPU oracle (oracle/mlp.py, pinned by torch autograd and finite differences), the GPU Adam
 step, the training loop, the dataset kernel against the C oracle's con2prim, and the A/B of
-plain against learned con2prim (what the trained network gains, and how often the judge
+plain against learned con2prim (what the trained network gains, and how often the acceptance test
 rejects it)."""
 import os
 
@@ -147,14 +147,14 @@ def _residual(P, U, q, gamma):
 
 def test_learned_guess_ab():
     """What the trained network gains: the same held-out states through plain and learned
-    con2prim.  Where the plain solve converges by Newton to a valid state, the judge must accept
+    con2prim.  Where the plain solve converges by Newton to a valid state, the acceptance test must pass
     the learned start for nearly every state and the learned path must need fewer iterations;
     elsewhere (bisection, hydro-guess fallback) both paths return the same state."""
     log("test_learned_guess_ab")
     ev, st = learned.evaluate(_trained_weights(), n=1 << 18, seed=123, return_states=True)
     for k, v in ev.items():
         log(f"  {k}: {v}")
-    assert ev["judge_accept_rate_newton_valid"] >= 0.95
+    assert ev["accept_rate_newton_valid"] >= 0.95
     assert ev["learned_iters_newton_valid"] < 0.6 * ev["plain_iters_newton_valid"]
     assert ev["learned_iters_per_state"] < ev["plain_iters_per_state"]
     assert ev["identical_outside_newton_valid"] >= 0.99

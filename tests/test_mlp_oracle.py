@@ -1,4 +1,4 @@
-"""Pins the CPU oracle of the learned-guess network (oracle/mlp.py) before it judges the GPU.
+"""Pins the CPU oracle of the learned-guess network (oracle/mlp.py) before the GPU is checked against it.
 
 The network has no counterpart in the reference, so the checks come from outside the
 oracle's own formulas: torch autograd on the CPU (an independent implementation of the
