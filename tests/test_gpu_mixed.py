@@ -66,3 +66,51 @@ def test_mixed_step_vs_oracle(phys, recon, lim, riem, rk):
         l1 = np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-300)
         log(f"  var {v}: L1 rel {l1:.2e}")
         assert l1 <= 1e-5, v
+
+
+@pytest.mark.parametrize("case", ["dissipation_muscl", "dissipation_ppm", "grhd_schwarzschild", "grmhd_kerr_schild"])
+def test_mixed_step_extensions_vs_oracle(case):
+    """The mixed path with the dissipative SRHD face hooks (fields 5..14 in FP64, added to the FP32
+    fluxes) and with the GR wrappers (the rescaled flux velocities in FP32 faces, the lapse
+    sources in the FP64 update), against the FP64 oracle, with the bars above."""
+    log(f"test_mixed_step_extensions_vs_oracle {case}")
+    ng = 3
+    dx = 1.0 / 22
+    dt = 0.4 * dx * 0.5
+    if case.startswith("dissipation"):
+        recon = case.split("_")[1]
+        base = smooth_state((22 + 2 * ng, 20 + 2 * ng, 18 + 2 * ng), 93, "hydro", vamp=0.5)
+        diss = np.random.default_rng(94).uniform(-0.004, 0.004, (10,) + base.shape[1:])
+        pr = np.concatenate([base[:5], diss])
+        d = make_domain("hydro", pr.shape, ng, dxyz=(dx, dx, dx), recon=recon, limiter="mc", riemann="hllc",
+                        rk_order=3, precision="mixed", passive_offset=15)
+        P = O.make_params("hydro", pr.shape[1:], nv=15, ng=ng, p_max=2.0, v_max=0.9999, limiter="mc", recon=recon,
+                          riemann="hllc", rk_order=3, passive_offset=15)
+        phys = "hydro"
+    else:
+        phys = "rmhd" if case.startswith("grmhd") else "hydro"
+        metric = "kerr-schild" if "kerr" in case else "schwarzschild"
+        pr = smooth_state((22 + 2 * ng, 20 + 2 * ng, 18 + 2 * ng), 95, phys, vamp=0.3)
+        gamma = 4.0 / 3.0 if phys == "rmhd" else None
+        g_gr = 4.0 / 3.0 if phys == "rmhd" else 1.4
+        d = make_domain(phys, pr.shape, ng, dxyz=(dx, dx, dx), gamma=gamma, recon="weno", limiter="mc",
+                        riemann="hlld_full" if phys == "rmhd" else "hllc", rk_order=3, precision="mixed",
+                        gr_metric=metric, gr_mass=0.05, gr_gamma=g_gr)
+        P = O.make_params("grmhd" if phys == "rmhd" else "hydro", pr.shape[1:], nv=pr.shape[0], ng=ng, gamma=gamma,
+                          p_max=2.0, v_max=0.9999, limiter="mc", recon="weno",
+                          riemann="hlld_full" if phys == "rmhd" else "hllc", rk_order=3, gr_metric=metric,
+                          gr_mass=0.05, gr_gamma=g_gr)
+    d.set_prim(pr)
+    d.step(dt)
+    out = d.get_prim()
+    d.close()
+    ref = O.step(P, pr, dx, dx, dx, dt)
+    core = (slice(ng, -ng),) * 3
+    for name, a, b in (("rho", out[0], ref[0]), ("p", out[4], ref[4]), ("W", _lorentz(out), _lorentz(ref))):
+        a, b = a[core], b[core]
+        l1 = np.abs(a - b).sum() / np.abs(b).sum()
+        mx = np.abs(a - b).max() / np.abs(b).max()
+        log(f"  {name}: L1 rel {l1:.2e}  max rel {mx:.2e}")
+        assert l1 <= (1e-6 if name == "rho" else 1e-5), name
+        assert mx <= (1e-3 if phys == "rmhd" else 1e-4), name
+    assert np.array_equal(out[5:15], pr[5:15]) if case.startswith("dissipation") else True
