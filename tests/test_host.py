@@ -164,3 +164,27 @@ def test_post_step_source_settings():
     assert pr.shape[0] == 11 and (pr[8] == 1.0).all() and (pr[9] == 0.2).all() and (pr[10] == 0.0).all()
     off = sources_from_settings(dict(s, COOLING_ENABLED=False, N_CHEM=2))
     assert off.cooling == 0 and off.chemistry == 0
+
+
+def test_params_for_gr_and_dissipation():
+    """PHYSICS grhd / grmhd map to the SRHD / RMHD context with the GR fields set (core/grhd_core.py:13-18);
+    DISSIPATION_ENABLED moves the passives to PASSIVE_OFFSET = 15 (utils/settings.py:250-252)."""
+    from astrofd_amd.domain import gr_metric_id, params_from_settings, sources_from_settings
+    s = finalize(dict(DEFAULTS, NX=8, NY=8, NZ=8, PHYSICS="grhd", GAMMA=1.4, GR_METRIC="Schwarzschild",
+                      GR_MASS=0.3, ORTHONORMAL_FLUX=False))
+    p = params_from_settings(s)
+    assert p.physics == 0 and p.gr_metric == 1 and p.gr_orthonormal == 0 and p.gr_mass == 0.3
+    assert p.gamma == 5.0 / 3.0 and p.gr_gamma == 1.4  # Q5: the SRHD core keeps 5/3, the GR source takes GAMMA
+    s = finalize(dict(DEFAULTS, NX=8, NY=8, NZ=8, PHYSICS="grmhd", GAMMA=4.0 / 3.0, GR_METRIC="kerr-schild"))
+    p = params_from_settings(s)
+    assert p.physics == 1 and p.gr_metric == 2 and p.gr_gamma == p.gamma == 4.0 / 3.0
+    assert gr_metric_id("minkowski") == 0 and gr_metric_id("kerr") == 3
+    s = finalize(dict(DEFAULTS, NX=8, NY=8, NZ=8, PHYSICS="hydro", DISSIPATION_ENABLED=True, N_TRACERS=1,
+                      BULK_ZETA=0.1, RELAX_TAU=0.2, RELAX_TAU_SHEAR=0.0))
+    p = params_from_settings(s)
+    assert p.passive_offset == 15 and p.n_passive == 1 and p.nv == 16
+    src = sources_from_settings(s)
+    assert src.dissipation == 1 and src.diss.zeta == 0.1 and src.diss.tau_bulk == 0.2 and src.diss.tau_shear == 0.2
+    assert ic.init_block(s).shape[0] == 16
+    with pytest.raises(NotImplementedError):
+        params_from_settings(dict(s, PHYSICS="mhd"))
