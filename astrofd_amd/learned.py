@@ -104,7 +104,7 @@ def build_dataset(dom, n, seed, chunk=1 << 22, slow_frac=0.5):
     return np.concatenate(Xs), np.concatenate(Ys), nvalid / n
 
 
-def train(dom, X, Y, epochs=60, batch=32768, lr0=3e-3, lr_min=3e-5, seed=11, w0=None, log=print):
+def train(dom, X, Y, epochs=200, batch=32768, lr0=3e-3, lr_min=3e-5, seed=11, w0=None, log=print):
     """Adam on the GPU with a cosine learning-rate schedule (one lr per epoch).
     Returns (weights, per-step loss history)."""
     rng = np.random.default_rng(seed)
@@ -191,7 +191,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--out", default=MODEL_PATH)
     ap.add_argument("--samples", type=int, default=1 << 24)
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=200)
     ap.add_argument("--batch", type=int, default=32768)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--seed", type=int, default=2024)
