@@ -229,6 +229,11 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
             return fail(AFD_ERR_INVALID, "grid too small for the reconstruction stencil");
         }
     }
+    for (int d = 0; d < 3; ++d)
+        if (face_count(g, d) >= (1L << 31)) {  // the face kernels index faces in 32 bits
+            delete c;
+            return fail(AFD_ERR_INVALID, "grid too large: 2^31 or more faces in one direction");
+        }
     c->carried = nb == 9 && p.recon_id != AFD_RECON_MUSCL;
     c->passive_flux = p.n_passive > 0 && (nb == 9 ? p.recon_id == AFD_RECON_MUSCL : p.recon_id != AFD_RECON_WENO);
     c->fsize = p.precision == AFD_PREC_MIXED ? 4 : 8;

@@ -43,6 +43,22 @@ inline dim3 grid1d(long n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
 
 AFD_D long cidx(const Grid &g, int i, int j, int k) { return ((long)i * g.ny + j) * g.nz + k; }
 
+// n / d for 0 <= n < 2^31 and a fixed 1 <= d < 2^31, as one multiply and one shift
+// (Granlund & Montgomery, "Division by invariant integers using multiplication", PLDI 1994).
+// With s = ceil(log2 d) and m = floor(2^(31+s) / d) + 1 (< 2^32), n / d = (n m) >> (31 + s):
+// m d = 2^(31+s) + e with 0 < e <= d <= 2^s, so n m / 2^(31+s) = n / d + n e / (d 2^(31+s)),
+// and n e < 2^(31+s) keeps the floor at n / d.
+struct UDiv {
+    unsigned m;
+    int sh;
+};
+inline UDiv make_udiv(unsigned d) {
+    int s = 0;
+    while ((1ull << s) < d) ++s;
+    return UDiv{(unsigned)((1ull << (31 + s)) / d + 1ull), 31 + s};
+}
+AFD_D unsigned udiv(const UDiv &q, unsigned n) { return (unsigned)(((unsigned long long)n * q.m) >> q.sh); }
+
 template <typename T> struct FaceArgs {
     const double *P;   // primitives of the stage input
     T *F;              // face flux (own speeds), nb x ncell
@@ -54,6 +70,7 @@ template <typename T> struct FaceArgs {
     double p_max, v_max;
     int lim;
     int diss;          // SRHD dissipative hooks: bit 0 bulk pressure on the face pressures, bits 1-2 flux repeats
+    UDiv dk, dn;       // face box: division by its z extent, then by its x (DIR 0) or y extent
 };
 
 // ------------------------------------------------------------------ reconstruction
@@ -109,25 +126,27 @@ AFD_D long xcd_chunked_block(long b, long nb) {
 // that those rows share an XCD and its L2.  (With the plain order the y stencil's rows landed on
 // six XCDs and the y kernels fetched 3x their compulsory bytes from HBM at 256^3.)  The z
 // stencil lies within a row.
-template <int DIR> AFD_D long face_thread() {
-    const long b = DIR != 2 ? xcd_chunked_block(blockIdx.x, gridDim.x) : (long)blockIdx.x;
+template <int DIR> AFD_D unsigned face_thread() {
+    const unsigned b = DIR != 2 ? (unsigned)xcd_chunked_block(blockIdx.x, gridDim.x) : blockIdx.x;
     return b * BLOCK + threadIdx.x;
 }
 
-template <int DIR> AFD_D bool face_coords(const Grid &g, long t, int &i, int &j, int &k) {
+// Face (i, j, k) of thread t; the launcher keeps the face count below 2^31.
+template <int DIR> AFD_D bool face_coords(const Grid &g, const UDiv &dk, const UDiv &dn, unsigned t, int &i, int &j,
+                                          int &k) {
     int n[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) n[a] = g.r1[a] - g.r0[a] + (a == DIR ? 1 : 0);
-    long tot = (long)n[0] * n[1] * n[2];
+    const unsigned tot = (unsigned)n[0] * n[1] * n[2];
     if (t >= tot) return false;
-    k = (int)(t % n[2]);
-    long r = t / n[2];
+    const unsigned r = udiv(dk, t), q = udiv(dn, r);
+    k = (int)(t - r * n[2]);
     if (DIR == 0) {  // k, then i (the normal), then j
-        i = (int)(r % n[0]);
-        j = (int)(r / n[0]);
+        i = (int)(r - q * n[0]);
+        j = (int)q;
     } else {         // k, then j, then i
-        j = (int)(r % n[1]);
-        i = (int)(r / n[1]);
+        j = (int)(r - q * n[1]);
+        i = (int)q;
     }
     i += g.r0[0];
     j += g.r0[1];
@@ -166,7 +185,7 @@ template <int RECON, int RIEM, int DIR, typename T>
 __global__ __launch_bounds__(BLOCK) void k_face_srhd(FaceArgs<T> a) {
     const Grid &g = a.g;
     int i, j, k;
-    if (!face_coords<DIR>(g, face_thread<DIR>(), i, j, k)) return;
+    if (!face_coords<DIR>(g, a.dk, a.dn, face_thread<DIR>(), i, j, k)) return;
     const long c = cidx(g, i, j, k), s = dstride<DIR>(g);
     double qLd[5], qRd[5];
 #pragma unroll
@@ -285,7 +304,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(FaceOcc<T
 void k_face_rmhd(FaceArgs<T> a) {
     const Grid &g = a.g;
     int i, j, k;
-    if (!face_coords<DIR>(g, face_thread<DIR>(), i, j, k)) return;
+    if (!face_coords<DIR>(g, a.dk, a.dn, face_thread<DIR>(), i, j, k)) return;
     const long c = cidx(g, i, j, k);
     RmhdFace<DIR, T> f;
     rmhd_face_setup<RECON, DIR, T>(a, c, f);

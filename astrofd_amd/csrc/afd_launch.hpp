@@ -33,6 +33,9 @@ template <typename T> inline FaceArgs<T> face_args(const FaceLaunch &l) {
     a.diss = l.diss;
     if constexpr (sizeof(T) == 8) a.ph = l.phd;
     else a.ph = l.phf;
+    const int *r0 = l.g.r0, *r1 = l.g.r1;
+    a.dk = make_udiv((unsigned)(r1[2] - r0[2] + (l.dir == 2 ? 1 : 0)));
+    a.dn = make_udiv((unsigned)(l.dir == 0 ? r1[0] - r0[0] + 1 : r1[1] - r0[1] + (l.dir == 1 ? 1 : 0)));
     return a;
 }
 
