@@ -214,6 +214,12 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
     g.sy = g.nz;
     g.sx = (long)g.ny * g.nz;
     g.ncell = (long)g.nx * g.ny * g.nz;
+    if (g.ncell >= (1L << 31)) {  // cells and faces are indexed in 32 bits inside the kernels
+        delete c;
+        return fail(AFD_ERR_INVALID, "grid too large: 2^31 or more cells with ghosts");
+    }
+    g.dnz = make_udiv((unsigned)g.nz);
+    g.dny = make_udiv((unsigned)g.ny);
     const int full[3] = {g.nx, g.ny, g.nz};
     for (int a = 0; a < 3; ++a) {
         // core/srhd_core.py:477-479 [2, n-3); core/rmhd_core.py:751-753 [2, n-2); PPM/WENO [3, n-3)
@@ -229,11 +235,6 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
             return fail(AFD_ERR_INVALID, "grid too small for the reconstruction stencil");
         }
     }
-    for (int d = 0; d < 3; ++d)
-        if (face_count(g, d) >= (1L << 31)) {  // the face kernels index faces in 32 bits
-            delete c;
-            return fail(AFD_ERR_INVALID, "grid too large: 2^31 or more faces in one direction");
-        }
     c->carried = nb == 9 && p.recon_id != AFD_RECON_MUSCL;
     c->passive_flux = p.n_passive > 0 && (nb == 9 ? p.recon_id == AFD_RECON_MUSCL : p.recon_id != AFD_RECON_WENO);
     c->fsize = p.precision == AFD_PREC_MIXED ? 4 : 8;

@@ -9,7 +9,8 @@ __global__ __launch_bounds__(BLOCK) void k_passive_rhs(PassiveArgs a) {
     const Grid &g = a.g;
     long c = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (c >= g.ncell) return;
-    int k = (int)(c % g.nz), j = (int)((c / g.nz) % g.ny), i = (int)(c / ((long)g.ny * g.nz));
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
     bool in = i >= g.r0[0] && i < g.r1[0] && j >= g.r0[1] && j < g.r1[1] && k >= g.r0[2] && k < g.r1[2];
     for (int t = 0; t < a.np; ++t) {
         double r = 0.0;
@@ -327,9 +328,8 @@ __global__ __launch_bounds__(BLOCK) void k_gr_rescale(GrRescaleArgs a) {
     if (c >= g.ncell) return;
     const long n = g.ncell;
     for (int v = 0; v < g.nv; ++v) a.Q[v * n + c] = a.P[v * n + c];
-    const int k = (int)(c % g.nz);
-    const long t = c / g.nz;
-    const int j = (int)(t % g.ny), i = (int)(t / g.ny);
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
     if (i < a.ng || i >= g.nx - a.ng || j < a.ng || j >= g.ny - a.ng || k < a.ng || k >= g.nz - a.ng) return;
     double X[3];
     gr_xyz(a.x0, a.ng, i, j, k, a.dx, a.dy, a.dz, X);

@@ -20,10 +20,27 @@ namespace afd {
 
 constexpr int BLOCK = 256;
 
+// n / d for 0 <= n < 2^31 and a fixed 1 <= d < 2^31, as one multiply and one shift
+// (Granlund & Montgomery, "Division by invariant integers using multiplication", PLDI 1994).
+// With s = ceil(log2 d) and m = floor(2^(31+s) / d) + 1 (< 2^32), n / d = (n m) >> (31 + s):
+// m d = 2^(31+s) + e with 0 < e <= d <= 2^s, so n m / 2^(31+s) = n / d + n e / (d 2^(31+s)),
+// and n e < 2^(31+s) keeps the floor at n / d.
+struct UDiv {
+    unsigned m;
+    int sh;
+};
+inline UDiv make_udiv(unsigned d) {
+    int s = 0;
+    while ((1ull << s) < d) ++s;
+    return UDiv{(unsigned)((1ull << (31 + s)) / d + 1ull), 31 + s};
+}
+AFD_D unsigned udiv(const UDiv &q, unsigned n) { return (unsigned)(((unsigned long long)n * q.m) >> q.sh); }
+
 struct Grid {
     int nv, nx, ny, nz;   // full dims (ghosts included)
     int r0[3], r1[3];     // cells that receive a flux divergence: [r0, r1) per axis
-    long sx, sy, ncell;   // strides of x and y, number of cells
+    long sx, sy, ncell;   // strides of x and y, number of cells (below 2^31)
+    UDiv dnz, dny;        // division by nz and by ny
 };
 
 // Minimum waves per SIMD the face kernels are compiled for (the register budget follows:
@@ -42,22 +59,12 @@ template <typename T> struct FaceOcc {
 inline dim3 grid1d(long n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
 
 AFD_D long cidx(const Grid &g, int i, int j, int k) { return ((long)i * g.ny + j) * g.nz + k; }
-
-// n / d for 0 <= n < 2^31 and a fixed 1 <= d < 2^31, as one multiply and one shift
-// (Granlund & Montgomery, "Division by invariant integers using multiplication", PLDI 1994).
-// With s = ceil(log2 d) and m = floor(2^(31+s) / d) + 1 (< 2^32), n / d = (n m) >> (31 + s):
-// m d = 2^(31+s) + e with 0 < e <= d <= 2^s, so n m / 2^(31+s) = n / d + n e / (d 2^(31+s)),
-// and n e < 2^(31+s) keeps the floor at n / d.
-struct UDiv {
-    unsigned m;
-    int sh;
-};
-inline UDiv make_udiv(unsigned d) {
-    int s = 0;
-    while ((1ull << s) < d) ++s;
-    return UDiv{(unsigned)((1ull << (31 + s)) / d + 1ull), 31 + s};
+AFD_D void cell_ijk(const Grid &g, long c, int &i, int &j, int &k) {
+    const unsigned r = udiv(g.dnz, (unsigned)c), q = udiv(g.dny, r);
+    k = (int)((unsigned)c - r * (unsigned)g.nz);
+    j = (int)(r - q * (unsigned)g.ny);
+    i = (int)q;
 }
-AFD_D unsigned udiv(const UDiv &q, unsigned n) { return (unsigned)(((unsigned long long)n * q.m) >> q.sh); }
 
 template <typename T> struct FaceArgs {
     const double *P;   // primitives of the stage input
@@ -131,7 +138,7 @@ template <int DIR> AFD_D unsigned face_thread() {
     return b * BLOCK + threadIdx.x;
 }
 
-// Face (i, j, k) of thread t; the launcher keeps the face count below 2^31.
+// Face (i, j, k) of thread t; a face box has fewer faces than the grid has cells (< 2^31).
 template <int DIR> AFD_D bool face_coords(const Grid &g, const UDiv &dk, const UDiv &dn, unsigned t, int &i, int &j,
                                           int &k) {
     int n[3];
@@ -498,9 +505,8 @@ AFD_D void gr_xyz(int x0, int ng, int i, int j, int k, double dx, double dy, dou
 template <int NB, typename F> AFD_D void gr_source(const UpdArgs<F> &a, long c, double *rhs) {
 #pragma clang fp contract(off)
     const Grid &g = a.g;
-    const int k = (int)(c % g.nz);
-    const long t = c / g.nz;
-    const int j = (int)(t % g.ny), i = (int)(t / g.ny);
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
     if (i < a.ng || i >= g.nx - a.ng || j < a.ng || j >= g.ny - a.ng || k < a.ng || k >= g.nz - a.ng) return;
     double X[3], d[3], q[NB], U[NB];
     gr_xyz(a.gr_x0, a.ng, i, j, k, a.dx, a.dy, a.dz, X);
@@ -576,7 +582,8 @@ template <int NB, typename F> AFD_D void cell_rhs(const UpdArgs<F> &a, long c, b
 }
 
 AFD_D bool in_range(const Grid &g, long c) {
-    int k = (int)(c % g.nz), j = (int)((c / g.nz) % g.ny), i = (int)(c / ((long)g.ny * g.nz));
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
     return i >= g.r0[0] && i < g.r1[0] && j >= g.r0[1] && j < g.r1[1] && k >= g.r0[2] && k < g.r1[2];
 }
 
