@@ -125,6 +125,33 @@ template <typename T> AFD_D T weno5_right(T qp2, T qp1, T q0, T qm1, T qm2) {
     return weno5_left(qp2, qp1, q0, qm1, qm2);
 }
 
+// Both WENO5 edge values of one cell, from the deviations of its four stencil neighbours from
+// the cell's own value (the cell's deviation is 0): hi = weno5_left (the cell's right edge),
+// lo = weno5_right (its left edge).  The mirror image has the same three smoothness indicators
+// in reverse order, so one set of indicators and reciprocals serves both edges, and each edge
+// is normalised once: sum(a_k p_k) / sum(a_k) equals sum(w_k p_k) with w_k = a_k / sum(a_k).
+AFD_D void weno5_cell_pair(float dm2, float dm1, float dp1, float dp2, float &lo, float &hi) {
+    const float eps = 1e-12f;
+    const float IS0 = (13.0f / 12.0f) * sq(dm2 - 2.0f * dm1) + 0.25f * sq(dm2 - 4.0f * dm1);
+    const float IS1 = (13.0f / 12.0f) * sq(dm1 + dp1) + 0.25f * sq(dm1 - dp1);
+    const float IS2 = (13.0f / 12.0f) * sq(dp2 - 2.0f * dp1) + 0.25f * sq(dp2 - 4.0f * dp1);
+    const float r0 = __builtin_amdgcn_rcpf(sq(eps + IS0));
+    const float r1 = __builtin_amdgcn_rcpf(sq(eps + IS1));
+    const float r2 = __builtin_amdgcn_rcpf(sq(eps + IS2));
+    // right edge: weights 0.1, 0.6, 0.3 on stencils (-2..0), (-1..1), (0..2)
+    const float h0 = (1.0f / 3.0f) * dm2 - (7.0f / 6.0f) * dm1;
+    const float h1 = (-1.0f / 6.0f) * dm1 + (1.0f / 3.0f) * dp1;
+    const float h2 = (5.0f / 6.0f) * dp1 - (1.0f / 6.0f) * dp2;
+    const float ha0 = 0.1f * r0, ha1 = 0.6f * r1, ha2 = 0.3f * r2;
+    hi = (ha0 * h0 + ha1 * h1 + ha2 * h2) * __builtin_amdgcn_rcpf(ha0 + ha1 + ha2);
+    // left edge: the mirror image, weights 0.1, 0.6, 0.3 on stencils (0..2), (-1..1), (-2..0)
+    const float l0 = (1.0f / 3.0f) * dp2 - (7.0f / 6.0f) * dp1;
+    const float l1 = (-1.0f / 6.0f) * dp1 + (1.0f / 3.0f) * dm1;
+    const float l2 = (5.0f / 6.0f) * dm1 - (1.0f / 6.0f) * dm2;
+    const float la0 = 0.1f * r2, la2 = 0.3f * r0;
+    lo = (la0 * l0 + ha1 * l1 + la2 * l2) * __builtin_amdgcn_rcpf(la0 + ha1 + la2);
+}
+
 // core/srhd_core.py:171-186
 template <typename T> AFD_D T hlle_scalar(T qL, T qR, T fL, T fR, T sL, T sR) {
     if (sL >= T(0)) return fL;

@@ -9,7 +9,24 @@
 
 namespace afd {
 
+// FP32 WENO5 z faces: one lane per cell, the left neighbour's right edge from lane - 1.
+template <int RIEM> static void go_weno_z_f32(const FaceLaunch &l, hipStream_t s) {
+    FaceArgs<float> a = face_args<float>(l);
+    const Grid &g = l.g;
+    const unsigned nz = g.r1[2] - g.r0[2], ny = g.r1[1] - g.r0[1], nx = g.r1[0] - g.r0[0];
+    const unsigned ncz = nz + 2, npos = ncz * ny * nx, waves = (npos - 1 + 62) / 63;
+    a.dk = make_udiv(ncz);
+    a.dn = make_udiv(ny);
+    hipLaunchKernelGGL((k_face_rmhd_weno_lanes<RIEM>), grid1d(64L * waves), dim3(BLOCK), 0, s, a);
+}
+
 template <int RECON, int RIEM, typename T> static void go(const FaceLaunch &l, hipStream_t s) {
+    if constexpr (RECON == 2 && sizeof(T) == 4) {
+        if (l.dir == 2 && l.g.ncell < (1L << 29)) {
+            go_weno_z_f32<RIEM>(l, s);
+            return;
+        }
+    }
     FaceArgs<T> a = face_args<T>(l);
     dim3 grid = grid1d(face_count(l.g, l.dir));
     if (l.dir == 0) hipLaunchKernelGGL((k_face_rmhd<RECON, RIEM, 0, T>), grid, dim3(BLOCK), 0, s, a);
