@@ -384,6 +384,55 @@ AFD_D void weno5_cell_edges(const double *__restrict__ b, unsigned x, unsigned s
                     (float)(ld32(b, x + 2 * s) - v0), l, h);
 }
 
+// x and y faces: a workgroup of ROW_WAVES waves covers 64 consecutive z positions of
+// ROW_WAVES consecutive cells along the normal, n0 - 1 .. n0 + ROW_WAVES - 2.  Each lane evaluates
+// its cell's two edges; the right edges go through LDS to the wave of the next cell, and waves
+// 1 .. ROW_WAVES - 1 evaluate the faces n0 .. n0 + ROW_WAVES - 2.  Wave 0 only supplies cell
+// n0 - 1.  Workgroups are ordered z chunk fastest, then group of faces along the normal, then the
+// other transverse axis, and walk XCD-contiguous chunks.  a.dk divides by the number of z chunks,
+// a.dn by the number of face groups per line.
+#ifndef AFD_ROW_WAVES
+#define AFD_ROW_WAVES 4
+#endif
+constexpr int ROW_WAVES = AFD_ROW_WAVES;
+
+template <int RIEM, int DIR>
+__global__ __launch_bounds__(64 * ROW_WAVES) __attribute__((amdgpu_waves_per_eu(FaceOcc<float>::waves)))
+void k_face_rmhd_weno_rows(FaceArgs<float> a) {
+    static_assert(DIR == 0 || DIR == 1, "rows of cells along x or y");
+    constexpr int FG = ROW_WAVES - 1;  // faces per group
+    __shared__ float edge[ROW_WAVES][9][64];
+    const Grid &g = a.g;
+    constexpr int O = DIR == 0 ? 1 : 0;  // the transverse axis other than z
+    const int n2 = g.r1[2] - g.r0[2], nkc = (n2 + 63) / 64, nn = g.r1[DIR] - g.r0[DIR] + 1, ng = (nn + FG - 1) / FG;
+    const unsigned b = (unsigned)xcd_chunked_block(blockIdx.x, gridDim.x);
+    const unsigned r = udiv(a.dk, b), q = udiv(a.dn, r);
+    const int kc = (int)(b - r * nkc), grp = (int)(r - q * ng), o = g.r0[O] + (int)q;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int kraw = kc * 64 + lane, n = g.r0[DIR] + FG * grp + w - 1;
+    const bool live = kraw < n2 && n <= g.r1[DIR];
+    const int k = g.r0[2] + min(kraw, n2 - 1), nc = min(n, g.r1[DIR]);  // in range for the loads
+    const unsigned s = (unsigned)dstride<DIR>(g);
+    const unsigned c = (unsigned)(DIR == 0 ? cidx(g, nc, o, k) : cidx(g, o, nc, k));
+    double qLd[9], qRd[9];
+#pragma unroll
+    for (int v = 0; v < 9; ++v) {
+        double vm1, v0;
+        float l, h;
+        weno5_cell_edges(a.P + v * g.ncell, c, s, vm1, v0, l, h);
+        edge[w][v][lane] = h;
+        qLd[v] = vm1;
+        qRd[v] = v0 + (double)l;
+    }
+    __syncthreads();
+    if (w == 0 || !live) return;
+#pragma unroll
+    for (int v = 0; v < 9; ++v) qLd[v] += (double)edge[w - 1][v][lane];
+    RmhdFace<DIR, float> fc;
+    rmhd_face_states<DIR, float>(a, qLd, qRd, fc);
+    rmhd_face_store<2, RIEM, DIR, float>(a, fc, DIR == 0 ? n : o, DIR == 0 ? o : n, k, (long)c);
+}
+
 // z faces: one lane per cell k = r0 - 1 .. r1 of a z line (ncz = r1 - r0 + 2 cells), the
 // lines of the interior i, j range in order; lane l takes the right edge of cell k - 1 from
 // lane l - 1.  Wave w covers line positions 63 w .. 63 w + 63 and writes the faces of lanes 1..63,
@@ -718,9 +767,11 @@ AFD_D bool cell_c2p(const UpdArgs<F> &a, const double *U, bool live, long c, dou
 // network's Newton start; a separate instantiation keeps its registers out of the plain kernel.
 // The learned instantiation is held to 3 waves per SIMD (<= 168 VGPRs, no spills): left alone
 // the compiler takes 190 registers (2 waves), and on the 256^3 RMHD WENO5 + HLLD step the
-// kernel ran 16 % slower that way (4 waves spills and is 6 % slower than 3).
+// kernel ran 16 % slower that way (4 waves spills and is 6 % slower than 3).  The plain one is
+// held to 5 (<= 96 VGPRs, no spills; 100 registers and 4 waves otherwise): 6 % faster in FP32,
+// 2.4 % in FP64 on that step (6 waves spills and gains nothing more).
 template <int NB, typename F, bool LEARNED>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LEARNED ? 3 : 1))) void k_update(
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LEARNED ? 3 : 5))) void k_update(
     UpdArgs<F> a) {
     const Grid &g = a.g;
     const long c = (long)blockIdx.x * BLOCK + threadIdx.x;

@@ -20,10 +20,25 @@ template <int RIEM> static void go_weno_z_f32(const FaceLaunch &l, hipStream_t s
     hipLaunchKernelGGL((k_face_rmhd_weno_lanes<RIEM>), grid1d(64L * waves), dim3(BLOCK), 0, s, a);
 }
 
+// FP32 WENO5 x and y faces: workgroups of ROW_WAVES cells along the normal sharing edges through LDS.
+template <int RIEM> static void go_weno_xy_f32(const FaceLaunch &l, hipStream_t s) {
+    FaceArgs<float> a = face_args<float>(l);
+    const Grid &g = l.g;
+    const unsigned nz = g.r1[2] - g.r0[2], nkc = (nz + 63) / 64;
+    const unsigned nn = g.r1[l.dir] - g.r0[l.dir] + 1, ng = (nn + ROW_WAVES - 2) / (ROW_WAVES - 1);
+    const unsigned no = l.dir == 0 ? g.r1[1] - g.r0[1] : g.r1[0] - g.r0[0];
+    a.dk = make_udiv(nkc);
+    a.dn = make_udiv(ng);
+    const dim3 grid(nkc * ng * no), block(64 * ROW_WAVES);
+    if (l.dir == 0) hipLaunchKernelGGL((k_face_rmhd_weno_rows<RIEM, 0>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_face_rmhd_weno_rows<RIEM, 1>), grid, block, 0, s, a);
+}
+
 template <int RECON, int RIEM, typename T> static void go(const FaceLaunch &l, hipStream_t s) {
     if constexpr (RECON == 2 && sizeof(T) == 4) {
-        if (l.dir == 2 && l.g.ncell < (1L << 29)) {
-            go_weno_z_f32<RIEM>(l, s);
+        if (l.g.ncell < (1L << 29)) {
+            if (l.dir == 2) go_weno_z_f32<RIEM>(l, s);
+            else go_weno_xy_f32<RIEM>(l, s);
             return;
         }
     }
