@@ -244,9 +244,17 @@ template <int DIR, typename T> struct RmhdFace {
     T pL, pR, vL, vR, sL, sR;
 };
 
-// Face fluxes and speeds from the reconstructed FP64 face states (modified in place by the floors).
+// The nine FP64 primitives of one face state.
+struct State9 {
+    double q[9];
+};
+
+// Face fluxes and speeds from the reconstructed FP64 face states.  The states come by value: with
+// them passed as pointers into the caller's arrays, the FP64 per-face kernels spilled 76-120 bytes
+// per lane at 3 waves per SIMD.
 template <int DIR, typename T>
-AFD_D void rmhd_face_states(const FaceArgs<T> &a, double *qLd, double *qRd, RmhdFace<DIR, T> &f) {
+AFD_D void rmhd_face_states(const FaceArgs<T> &a, State9 L, State9 R, RmhdFace<DIR, T> &f) {
+    double *qLd = L.q, *qRd = R.q;
     rmhd_floor(a.p_max, a.v_max, qLd);
     rmhd_floor(a.p_max, a.v_max, qRd);
     double vcL[3], vcR[3];
@@ -287,10 +295,10 @@ template <int RECON, int DIR, typename T>
 AFD_D void rmhd_face_setup(const FaceArgs<T> &a, long c, RmhdFace<DIR, T> &f) {
     const Grid &g = a.g;
     const long s = dstride<DIR>(g);
-    double qLd[9], qRd[9];
+    State9 L, R;
 #pragma unroll
-    for (int v = 0; v < 9; ++v) recon_face<RECON, T>(a.P + v * g.ncell, c, s, a.lim, qLd[v], qRd[v]);
-    rmhd_face_states<DIR, T>(a, qLd, qRd, f);
+    for (int v = 0; v < 9; ++v) recon_face<RECON, T>(a.P + v * g.ncell, c, s, a.lim, L.q[v], R.q[v]);
+    rmhd_face_states<DIR, T>(a, L, R, f);
 }
 
 // core/rmhd_core.py:569-578, 726-746 and the MUSCL-only hlld override (:811-831)
@@ -414,22 +422,22 @@ void k_face_rmhd_weno_rows(FaceArgs<float> a) {
     const int k = g.r0[2] + min(kraw, n2 - 1), nc = min(n, g.r1[DIR]);  // in range for the loads
     const unsigned s = (unsigned)dstride<DIR>(g);
     const unsigned c = (unsigned)(DIR == 0 ? cidx(g, nc, o, k) : cidx(g, o, nc, k));
-    double qLd[9], qRd[9];
+    State9 L, R;
 #pragma unroll
     for (int v = 0; v < 9; ++v) {
         double vm1, v0;
         float l, h;
         weno5_cell_edges(a.P + v * g.ncell, c, s, vm1, v0, l, h);
         edge[w][v][lane] = h;
-        qLd[v] = vm1;
-        qRd[v] = v0 + (double)l;
+        L.q[v] = vm1;
+        R.q[v] = v0 + (double)l;
     }
     __syncthreads();
     if (w == 0 || !live) return;
 #pragma unroll
-    for (int v = 0; v < 9; ++v) qLd[v] += (double)edge[w - 1][v][lane];
+    for (int v = 0; v < 9; ++v) L.q[v] += (double)edge[w - 1][v][lane];
     RmhdFace<DIR, float> fc;
-    rmhd_face_states<DIR, float>(a, qLd, qRd, fc);
+    rmhd_face_states<DIR, float>(a, L, R, fc);
     rmhd_face_store<2, RIEM, DIR, float>(a, fc, DIR == 0 ? n : o, DIR == 0 ? o : n, k, (long)c);
 }
 
@@ -451,18 +459,18 @@ void k_face_rmhd_weno_lanes(FaceArgs<float> a) {
     const unsigned line = udiv(a.dk, pos), kk = pos - line * ncz, ii = udiv(a.dn, line);
     const int i = g.r0[0] + (int)ii, j = g.r0[1] + (int)(line - ii * ny), k = g.r0[2] - 1 + (int)kk;
     const unsigned c = (unsigned)cidx(g, i, j, k);
-    double qLd[9], qRd[9];
+    State9 L, R;
 #pragma unroll
     for (int v = 0; v < 9; ++v) {
         double vm1, v0;
         float l, h;
         weno5_cell_edges(a.P + v * g.ncell, c, 1u, vm1, v0, l, h);
-        qLd[v] = vm1 + (double)__shfl_up(h, 1);
-        qRd[v] = v0 + (double)l;
+        L.q[v] = vm1 + (double)__shfl_up(h, 1);
+        R.q[v] = v0 + (double)l;
     }
     if (!valid || lane == 0 || kk == 0) return;
     RmhdFace<2, float> fc;
-    rmhd_face_states<2, float>(a, qLd, qRd, fc);
+    rmhd_face_states<2, float>(a, L, R, fc);
     rmhd_face_store<2, RIEM, 2, float>(a, fc, i, j, k, c);
 }
 

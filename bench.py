@@ -199,17 +199,22 @@ def kernel_name(s, which, mixed, learned):
         return f"k_update<{nb}, {T}, {'true' if learned else 'false'}>"
     from astrofd_amd.domain import recon_id, riemann_id
     d = {"face_x": 0, "face_y": 1, "face_z": 2}[which]
+    rid = riemann_id(s['PHYSICS'], s['RIEMANN'])
+    cells = (s["NX"] + 2 * s["NG"]) * (s["NY"] + 2 * s["NG"]) * (s["NZ"] + 2 * s["NG"])
+    if nb == 9 and mixed and recon_id(s['RECON']) == 2 and cells < (1 << 29):
+        # FP32 WENO5 faces sharing each cell's edge values (LDS rows for x/y, lane shuffle for z)
+        return f"k_face_rmhd_weno_lanes<{rid}>" if d == 2 else f"k_face_rmhd_weno_rows<{rid}, {d}>"
     kind = "k_face_rmhd" if nb == 9 else "k_face_srhd"
-    return f"{kind}<{recon_id(s['RECON'])}, {riemann_id(s['PHYSICS'], s['RIEMANN'])}, {d}, {T}>"
+    return f"{kind}<{recon_id(s['RECON'])}, {rid}, {d}, {T}>"
 
 
 def kernel_roofline(s, kern, mixed, learned=False, workload=None):
     """Dominant kernel and its achieved algorithmic HBM bandwidth (bytes per launch / avg duration).
 
     Algorithmic bytes per launch (compulsory traffic of one launch over its units):
-      face kernel, one thread per face: read the nb primitives of one cell (8 B each; the stencil's
-        other cells are re-reads) and write the face flux (nb x flux size; twice for the RMHD PPM/WENO
-        x faces, which also read the two carried speeds the z kernel wrote there).
+      face kernel, per face: read the nb primitives of one cell (8 B each; the stencil's other cells
+        are re-reads) and write the face flux (nb x flux size; twice for the RMHD PPM/WENO x faces,
+        which also read the two carried speeds the z kernel wrote there).
       stage update, one thread per cell (ghosts included): per stage 4 x nb x 8 B of conserved and
         primitive state (stage 0: read P0, write U0, U1, P1; later stages: read U0, U_s-1, write U_s,
         P_s), the psi of the stage input (8 B) and one read of each face-flux array (3 arrays, 4 with
