@@ -32,9 +32,25 @@ CASES = [
 @pytest.mark.parametrize("phys,recon,lim,riem,rk", CASES)
 def test_mixed_step_vs_oracle(phys, recon, lim, riem, rk):
     log(f"test_mixed_step_vs_oracle {phys} {recon} {lim} {riem} rk{rk}")
+    check_mixed_step(phys, recon, lim, riem, rk, (22, 20, 18))
+
+
+@pytest.mark.parametrize("interior", [(19, 16, 70), (21, 18, 130), (2, 3, 2)])
+def test_mixed_weno_shared_edges_shapes(interior):
+    """The FP32 WENO5 RMHD faces that share cell edges (4-wave LDS rows for x/y, lane shuffles for
+    z) on interior extents that leave partial 64-lane z chunks (70, 130), a last group along the
+    normal of 2 or 1 faces (20 and 17 x/y faces; 22 and 19), and partial last waves, against the
+    FP64 oracle with the bars above.  |v| <= 0.2 as in the FP64 parity tests: the modes span a
+    fixed number of periods per axis, and at 0.5 on these extents hlld_full's branch thresholds make
+    the per-face FP32 kernels miss the bars as well (L1(Bz) 3e-4)."""
+    log(f"test_mixed_weno_shared_edges_shapes {interior}")
+    check_mixed_step("rmhd", "weno", "mc", "hlld_full", 3, interior, vamp=0.2)
+
+
+def check_mixed_step(phys, recon, lim, riem, rk, interior, vamp=0.5):
     ng = 3
-    shape = (22 + 2 * ng, 20 + 2 * ng, 18 + 2 * ng)
-    pr = smooth_state(shape, 91, phys, vamp=0.5)
+    shape = tuple(n + 2 * ng for n in interior)
+    pr = smooth_state(shape, 91, phys, vamp=vamp)
     gamma = 4.0 / 3.0 if phys == "rmhd" else None
     dx = 1.0 / 22
     dt = 0.4 * dx * 0.5
