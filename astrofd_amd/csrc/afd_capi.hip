@@ -179,6 +179,9 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
     int nb = p.physics == AFD_PHYS_RMHD ? 9 : 5;
     if (p.ng < 2) return fail(AFD_ERR_INVALID, "NG must be >= 2");
     if (p.nx < 2 || p.ny < 2 || p.nz < 2) return fail(AFD_ERR_INVALID, "grid too small");
+    // cells and faces are indexed in 32 bits inside the kernels
+    if ((int64_t)(p.nx + 2 * p.ng) * (p.ny + 2 * p.ng) * (p.nz + 2 * p.ng) >= (int64_t(1) << 31))
+        return fail(AFD_ERR_INVALID, "grid too large: 2^31 or more cells with ghosts");
     // passive_offset < nb means "right after the base variables"; SRHD may hold fields between
     // the base variables and the passives (the dissipative Pi, pi^ij, q^i at 5..14 with
     // PASSIVE_OFFSET = 15, utils/settings.py:250-252), RMHD may not
@@ -214,10 +217,6 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
     g.sy = g.nz;
     g.sx = (long)g.ny * g.nz;
     g.ncell = (long)g.nx * g.ny * g.nz;
-    if (g.ncell >= (1L << 31)) {  // cells and faces are indexed in 32 bits inside the kernels
-        delete c;
-        return fail(AFD_ERR_INVALID, "grid too large: 2^31 or more cells with ghosts");
-    }
     g.dnz = make_udiv((unsigned)g.nz);
     g.dny = make_udiv((unsigned)g.ny);
     const int full[3] = {g.nx, g.ny, g.nz};

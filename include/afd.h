@@ -125,7 +125,9 @@ typedef struct afd_ctx afd_ctx;
 
 /* Context: replaces configure(params) (core/srhd_core.py:18, core/rmhd_core.py:21)
  * and the module globals it sets, plus init_block's allocation
- * (solvers/srhd3d_mpi_muscl.py:128-163). */
+ * (solvers/srhd3d_mpi_muscl.py:128-163).  AFD_ERR_INVALID for grids too small for the
+ * reconstruction stencil and for grids of 2^31 or more cells, ghosts included (the kernels
+ * index cells in 32 bits). */
 int afd_create(const afd_params *params, afd_ctx **out);
 int afd_destroy(afd_ctx *ctx);
 
