@@ -47,21 +47,29 @@ def test_mixed_weno_shared_edges_shapes(interior):
     check_mixed_step("rmhd", "weno", "mc", "hlld_full", 3, interior, vamp=0.2)
 
 
-def check_mixed_step(phys, recon, lim, riem, rk, interior, vamp=0.5):
+@pytest.mark.parametrize("phys,recon,lim,riem", [("rmhd", "weno", "mc", "hlld_full"), ("hydro", "muscl", "vanleer", "hllc")])
+def test_mixed_step_passives_vs_oracle(phys, recon, lim, riem):
+    """Two passive scalars: their FP64 fluxes take the FP32 faces' HLL speeds and face velocities
+    (written by the shared-edge WENO5 kernels for RMHD), against the FP64 oracle."""
+    log(f"test_mixed_step_passives_vs_oracle {phys} {recon}")
+    check_mixed_step(phys, recon, lim, riem, 3, (22, 20, 18), vamp=0.2, npass=2)
+
+
+def check_mixed_step(phys, recon, lim, riem, rk, interior, vamp=0.5, npass=0):
     ng = 3
     shape = tuple(n + 2 * ng for n in interior)
-    pr = smooth_state(shape, 91, phys, vamp=vamp)
+    pr = smooth_state(shape, 91, phys, vamp=vamp, npass=npass)
     gamma = 4.0 / 3.0 if phys == "rmhd" else None
     dx = 1.0 / 22
     dt = 0.4 * dx * 0.5
     d = make_domain(phys, pr.shape, ng, dxyz=(dx, dx, dx), gamma=gamma, recon=recon, limiter=lim, riemann=riem,
-                    rk_order=rk, precision="mixed")
+                    rk_order=rk, precision="mixed", n_passive=npass)
     d.set_prim(pr)
     d.step(dt)
     out = d.get_prim()
     d.close()
     P = O.make_params(phys, pr.shape[1:], nv=pr.shape[0], ng=ng, gamma=gamma, p_max=2.0, v_max=0.9999,
-                      limiter=lim, recon=recon, riemann=riem, rk_order=rk)
+                      limiter=lim, recon=recon, riemann=riem, rk_order=rk, n_passive=npass)
     ref = O.step(P, pr, dx, dx, dx, dt)
     core = (slice(ng, -ng),) * 3
     for name, a, b in (("rho", out[0], ref[0]), ("p", out[4], ref[4]), ("W", _lorentz(out), _lorentz(ref))):
@@ -76,7 +84,8 @@ def check_mixed_step(phys, recon, lim, riem, rk, interior, vamp=0.5):
         # D* <= SMALL, sign tests): an FP32 rounding next to one switches a face's branch, a
         # local jump of dt/dx times the difference of two consistent fluxes
         assert mx <= (1e-3 if riem == "hlld_full" else 1e-4), name
-    # B and psi: the same bar as p (their fluxes come out of the FP32 Riemann solve as well)
+    # B, psi and passives: the same bar as p (their fluxes come out of the FP32 Riemann solve, or
+    # use its speeds and face velocities)
     for v in range(5, pr.shape[0]):
         a, b = out[v][core], ref[v][core]
         l1 = np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-300)
