@@ -773,13 +773,14 @@ AFD_D bool cell_c2p(const UpdArgs<F> &a, const double *U, bool live, long c, dou
 // Stage update of one cell: flux divergence + sources, SSPRK combination into Uc (FP64), the
 // passives on primitives, and the con2prim of the new state into Pout.  LEARNED selects the
 // network's Newton start; a separate instantiation keeps its registers out of the plain kernel.
-// The learned instantiation is held to 3 waves per SIMD (<= 168 VGPRs, no spills): left alone
-// the compiler takes 190 registers (2 waves), and on the 256^3 RMHD WENO5 + HLLD step the
-// kernel ran 16 % slower that way (4 waves spills and is 6 % slower than 3).  The plain one is
-// held to 5 (<= 96 VGPRs, no spills; 100 registers and 4 waves otherwise): 6 % faster in FP32,
-// 2.4 % in FP64 on that step (6 waves spills and gains nothing more).
+// The learned instantiation is held to 4 waves per SIMD (<= 128 VGPRs; it takes 121, no
+// spills, with the network's two tiles evaluated one after the other): on the 256^3 RMHD
+// WENO5 + HLLD step it ran 16 % slower at the compiler's own 2 waves, and 4 % slower at 3 waves
+// with the tiles interleaved; 5 waves spills and is 25 % slower.  The plain one is held to 5
+// (<= 96 VGPRs, no spills; 100 registers and 4 waves otherwise): 6 % faster in FP32, 2.4 % in
+// FP64 on that step (6 waves spills and gains nothing more).
 template <int NB, typename F, bool LEARNED>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LEARNED ? 3 : 5))) void k_update(
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LEARNED ? 4 : 5))) void k_update(
     UpdArgs<F> a) {
     const Grid &g = a.g;
     const long c = (long)blockIdx.x * BLOCK + threadIdx.x;

@@ -155,6 +155,9 @@ AFD_D void mlp_infer_wave(const MlpDevW &w, const float x[5], float y[2]) {
     for (int f = 0; f < 5; ++f) xs[f] = __shfl_xor(x[f], 32);  // tile 1: samples 32..63 to lanes 0..31
     MlpTile t0, t1;
     mlp_tile_forward(w, lane, input_operand(x, upper), t0);
+    // one tile after the other: interleaved by the scheduler, the two tiles' activations took the
+    // stage update to 163 registers (3 waves per SIMD); one at a time it needs 121 (4 waves)
+    __builtin_amdgcn_sched_barrier(0);
     mlp_tile_forward(w, lane, input_operand(xs, upper), t1);
     float a0 = __shfl_xor(t1.o[0], 32), a1 = __shfl_xor(t1.o[1], 32);
     y[0] = upper ? a0 : t0.o[0];
