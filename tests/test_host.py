@@ -188,3 +188,25 @@ def test_params_for_gr_and_dissipation():
     assert ic.init_block(s).shape[0] == 16
     with pytest.raises(NotImplementedError):
         params_from_settings(dict(s, PHYSICS="mhd"))
+
+
+def test_multiply_shift_division_is_exact():
+    """The kernels' index division (UDiv in csrc/afd_kernels.hpp): with s = ceil(log2 d) and
+    m = floor(2^(31+s) / d) + 1, (n m) >> (31 + s) equals n / d for every 0 <= n < 2^31, and m fits
+    in 32 bits.  Checked by brute force where an error would first show: at both sides of every
+    multiple of d near 0 and near 2^31, for all d <= 4096 and for sampled d up to 2^31 - 1."""
+    rng = np.random.default_rng(5)
+    ds = list(range(1, 4097)) + [int(x) for x in rng.integers(4097, 2**31 - 1, 400)] + [2**31 - 1, 2**30, 2**30 + 1]
+    top = np.int64(2**31 - 1)
+    for d in ds:
+        s = 0
+        while (1 << s) < d:
+            s += 1
+        m = (1 << (31 + s)) // d + 1
+        assert m < 2**32
+        qs = np.unique(np.concatenate([np.arange(0, 64), top // d - np.arange(0, 64)]))
+        qs = qs[qs >= 0].astype(object)
+        n = np.concatenate([qs * d, qs * d - 1, qs * d + d - 1])
+        n = np.array([int(x) for x in n if 0 <= x <= int(top)], dtype=object)
+        got = (n * m) >> (31 + s)
+        assert np.array_equal(got, n // d), d
