@@ -374,9 +374,11 @@ void k_face_rmhd(FaceArgs<T> a) {
 // The mixed path's WENO5 z faces evaluate each cell's two edge values together
 // (weno5_cell_pair), as deviations from the cell's own value.  Face (c-1 | c) takes the right
 // edge of cell c-1 and the left edge of cell c.  Used for grids below 2^29 cells (32-bit byte
-// offsets within a field); larger grids take the per-face kernel.  (The x and y faces keep one
-// thread per face: a thread walking a run of faces along the normal, carrying the right edge,
-// ran 10-15 % slower on the 256^3 RMHD step despite 20-28 % fewer VALU instructions per face.)
+// offsets within a field); larger grids take the per-face kernel.  (A thread walking a run of
+// faces along the normal, carrying the right edge, ran 10-15 % slower on the 256^3 RMHD step
+// despite 20-28 % fewer VALU instructions per face.  The PPM and MUSCL faces keep the per-face
+// kernels: with one cheap reconstruction per cell to share, the same LDS rows and lane shuffles
+// ran 4-20 % slower per direction there.)
 
 // Element e of a field whose byte offsets fit in 32 bits: the load takes the field's base from
 // scalar registers and one 32-bit offset, shared by the nine fields.

@@ -35,16 +35,18 @@ def test_mixed_step_vs_oracle(phys, recon, lim, riem, rk):
     check_mixed_step(phys, recon, lim, riem, rk, (22, 20, 18))
 
 
+@pytest.mark.parametrize("recon,lim,riem", [("weno", "mc", "hlld_full"), ("ppm", "mc", "hlld"), ("muscl", "vanleer", "hlld")])
 @pytest.mark.parametrize("interior", [(19, 16, 70), (21, 18, 130), (2, 3, 2)])
-def test_mixed_weno_shared_edges_shapes(interior):
-    """The FP32 WENO5 RMHD faces that share cell edges (4-wave LDS rows for x/y, lane shuffles for
-    z) on interior extents that leave partial 64-lane z chunks (70, 130), a last group along the
-    normal of 2 or 1 faces (20 and 17 x/y faces; 22 and 19), and partial last waves, against the
-    FP64 oracle with the bars above.  |v| <= 0.2 as in the FP64 parity tests: the modes span a
-    fixed number of periods per axis, and at 0.5 on these extents hlld_full's branch thresholds make
-    the per-face FP32 kernels miss the bars as well (L1(Bz) 3e-4)."""
-    log(f"test_mixed_weno_shared_edges_shapes {interior}")
-    check_mixed_step("rmhd", "weno", "mc", "hlld_full", 3, interior, vamp=0.2)
+def test_mixed_shared_edges_shapes(interior, recon, lim, riem):
+    """The FP32 RMHD faces on interior extents that leave partial 64-lane z chunks (70, 130), a last
+    group along the normal of 2 or 1 faces, and partial last waves (the WENO5 faces share cell
+    edges through 4-wave LDS rows for x/y and lane shuffles for z; PPM and MUSCL run one thread
+    per face), against the FP64 oracle with the bars above.
+    |v| <= 0.2 as in the FP64 parity tests: the modes span a fixed number of periods per axis,
+    and at 0.5 on these extents hlld_full's branch thresholds make the per-face FP32 kernels miss
+    the bars as well (L1(Bz) 3e-4)."""
+    log(f"test_mixed_shared_edges_shapes {interior} {recon} {lim} {riem}")
+    check_mixed_step("rmhd", recon, lim, riem, 3, interior, vamp=0.2)
 
 
 @pytest.mark.parametrize("phys,recon,lim,riem", [("rmhd", "weno", "mc", "hlld_full"), ("hydro", "muscl", "vanleer", "hllc")])
