@@ -38,6 +38,7 @@ class Params(ctypes.Structure):
         ("two_temperature", _i32), ("thermo_offset", _i32), ("tei_tau", _dbl),
         ("device", _i32),
         ("gr_metric", _i32), ("gr_orthonormal", _i32), ("gr_x0", _i32), ("gr_mass", _dbl), ("gr_gamma", _dbl),
+        ("face_path", _i32),
     ]
 
 

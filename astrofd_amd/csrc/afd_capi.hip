@@ -204,6 +204,7 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
     if (p.n_resist < 0 || p.n_resist > 2) return fail(AFD_ERR_INVALID, "n_resist must be 0, 1 or 2");
     if (p.gr_metric < 0 || p.gr_metric > 3) return fail(AFD_ERR_INVALID, "gr_metric must be in [0, 3]");
     if (p.gr_metric != 0 && !(p.gr_gamma > 1.0)) return fail(AFD_ERR_INVALID, "gr_gamma must be > 1");
+    if (p.face_path != 0 && p.face_path != 1) return fail(AFD_ERR_INVALID, "face_path must be 0 or 1");
 
     HIPCHK(hipSetDevice(p.device));
     afd_ctx *c = new afd_ctx();
@@ -541,6 +542,7 @@ static FaceLaunch face_launch(afd_ctx *c, const double *P) {
     l.dir = 0;
     l.diss = c->diss;
     l.mixed = c->p.precision == AFD_PREC_MIXED;
+    l.shared_edges = c->p.face_path == 0;
     return l;
 }
 

@@ -17,6 +17,7 @@ struct FaceLaunch {
     double p_max, v_max;
     int lim, recon, riemann, dir, diss;
     bool mixed;
+    bool shared_edges;  // FP32 RMHD WENO5: the shared-edge kernels (afd_params.face_path == 0)
 };
 
 template <typename T> inline FaceArgs<T> face_args(const FaceLaunch &l) {

@@ -36,7 +36,7 @@ template <int RIEM> static void go_weno_xy_f32(const FaceLaunch &l, hipStream_t 
 
 template <int RECON, int RIEM, typename T> static void go(const FaceLaunch &l, hipStream_t s) {
     if constexpr (RECON == 2 && sizeof(T) == 4) {
-        if (l.g.ncell < (1L << 29)) {
+        if (l.shared_edges && l.g.ncell < (1L << 29)) {
             if (l.dir == 2) go_weno_z_f32<RIEM>(l, s);
             else go_weno_xy_f32<RIEM>(l, s);
             return;

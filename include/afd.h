@@ -77,6 +77,9 @@ typedef struct afd_params {
     int32_t gr_x0;            /* global x index of the first interior cell (0 on one GPU) */
     double gr_mass;           /* GR_MASS */
     double gr_gamma;          /* Gamma of the sources' enthalpy (GRHD: the configured GAMMA) */
+    int32_t face_path;        /* 0: automatic (the FP32 RMHD WENO5 faces share cell edge values on grids
+                                 below 2^29 cells); 1: one thread per face everywhere.  Same results to
+                                 FP32 rounding; no counterpart in the reference */
 } afd_params;
 
 /* Nozzle inflow on the -x ghost planes (core/nozzle.py:27-92).  The library's host code

@@ -9,7 +9,7 @@ def make_domain(physics, pr_shape, ng, dxyz=(0.1, 0.1, 0.1), gamma=None, p_max=2
                 recon="muscl", riemann="hlle", rk_order=2, n_passive=0, n_resist=0, resistivity=0.0,
                 precision="fp64", glm_ch=1.0, glm_cp=0.1, bc_y="periodic", bc_z="periodic",
                 two_temperature=False, thermo_offset=None, tei_tau=0.1, c2p_guess="plain", passive_offset=None,
-                gr_metric="minkowski", gr_mass=1.0, orthonormal_flux=True, gr_gamma=None):
+                gr_metric="minkowski", gr_mass=1.0, orthonormal_flux=True, gr_gamma=None, face_path=0):
     rm = physics == "rmhd"
     nb = 9 if rm else 5
     p = L.Params()
@@ -36,6 +36,7 @@ def make_domain(physics, pr_shape, ng, dxyz=(0.1, 0.1, 0.1), gamma=None, p_max=2
     p.thermo_offset = thermo_offset if thermo_offset is not None else nb
     p.tei_tau = tei_tau
     p.device = 0
+    p.face_path = face_path
     from astrofd_amd.domain import gr_metric_id
     p.gr_metric = gr_metric_id(gr_metric)
     p.gr_orthonormal = int(bool(orthonormal_flux))

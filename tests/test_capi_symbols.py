@@ -74,6 +74,7 @@ def test_parameter_validation_without_a_device():
         (dict(gr_metric=1, gr_gamma=1.0), b"gr_gamma"),
         (dict(nv=40, n_passive=35), b"nv > 32"),
         (dict(nx=1285, ny=1285, nz=1285), b"grid too large"),  # 1291^3 >= 2^31 cells with ghosts
+        (dict(face_path=2), b"face_path"),
     ]
     for kw, msg in cases:
         h = ctypes.c_void_p()

@@ -57,7 +57,15 @@ def test_mixed_step_passives_vs_oracle(phys, recon, lim, riem):
     check_mixed_step(phys, recon, lim, riem, 3, (22, 20, 18), vamp=0.2, npass=2)
 
 
-def check_mixed_step(phys, recon, lim, riem, rk, interior, vamp=0.5, npass=0):
+@pytest.mark.parametrize("riem", ["hlld_full", "hlld"])
+def test_mixed_per_face_fp32_faces(riem):
+    """afd_params.face_path = 1: the FP32 WENO5 RMHD faces with one thread per face, the path that
+    grids of 2^29 cells or more take, against the FP64 oracle with the bars above."""
+    log(f"test_mixed_per_face_fp32_faces {riem}")
+    check_mixed_step("rmhd", "weno", "mc", riem, 3, (22, 20, 18), face_path=1)
+
+
+def check_mixed_step(phys, recon, lim, riem, rk, interior, vamp=0.5, npass=0, face_path=0):
     ng = 3
     shape = tuple(n + 2 * ng for n in interior)
     pr = smooth_state(shape, 91, phys, vamp=vamp, npass=npass)
@@ -65,7 +73,7 @@ def check_mixed_step(phys, recon, lim, riem, rk, interior, vamp=0.5, npass=0):
     dx = 1.0 / 22
     dt = 0.4 * dx * 0.5
     d = make_domain(phys, pr.shape, ng, dxyz=(dx, dx, dx), gamma=gamma, recon=recon, limiter=lim, riemann=riem,
-                    rk_order=rk, precision="mixed", n_passive=npass)
+                    rk_order=rk, precision="mixed", n_passive=npass, face_path=face_path)
     d.set_prim(pr)
     d.step(dt)
     out = d.get_prim()
