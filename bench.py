@@ -13,6 +13,8 @@ Timed runs from the same seeded initial state:
   mixed  FP32 reconstruction + Riemann fluxes, FP64 Lorentz factor / con2prim /
          SSPRK accumulation, learned con2prim guess when weights are available
   mixed with the plain con2prim (what the learned guess changes in the step)
+  mixed with the previous-state Newton start (RMHD)
+  mixed with one thread per face (--face-path-ab: what the shared-edge face kernels change)
 `value` is the mixed run's zone-updates/s.  A lockstep pass of plain and mixed
 reports the relative L1 and max error of rho, p and W after checkpoint steps.
 
@@ -94,9 +96,9 @@ def make_stepper(d, s, rng):
     return one_step
 
 
-def open_domain(s, pr0, mlp=None):
+def open_domain(s, pr0, mlp=None, face_path=0):
     from astrofd_amd.domain import Domain
-    d = Domain.from_settings(s)
+    d = Domain.from_settings(s, face_path=face_path)
     d.set_prim(pr0)
     d.set_nozzle(s)
     if mlp is not None:
@@ -106,14 +108,14 @@ def open_domain(s, pr0, mlp=None):
     return d, rng
 
 
-def run_timed(s, pr0, steps, warmup, mlp=None, repeat=True):
+def run_timed(s, pr0, steps, warmup, mlp=None, repeat=True, face_path=0):
     """Time `steps` steps after `warmup` untimed ones, with per-kernel HIP events on the library's
     stream, then one untimed step with the solver counters on.
 
     repeat=True: every step starts from the seeded initial state (re-uploaded and filled
     outside the timed interval), so each timed step is one pass of the hot path over the same
     synthetic input.  repeat=False: consecutive steps of one run."""
-    d, rng = open_domain(s, pr0, mlp)
+    d, rng = open_domain(s, pr0, mlp, face_path)
     try:
         one_step = make_stepper(d, s, rng)
 
@@ -376,6 +378,8 @@ def main():
     ap.add_argument("--sources-only", action="store_true",
                     help="measure only the post-step source kernel and print its JSON line (profiling)")
     ap.add_argument("--accuracy-steps", default="1,2,5,10")
+    ap.add_argument("--face-path-ab", action="store_true",
+                    help="also time the mixed run with one thread per face (afd_params.face_path = 1)")
     ap.add_argument("--evolve", action="store_true",
                     help="time consecutive steps of one run instead of steps from the seeded state")
     ap.add_argument("--evolve-steps", type=int, default=10,
@@ -429,6 +433,12 @@ def main():
         mixed_prev = run_timed(dict(sm, C2P_GUESS="previous"), pr0, a.steps, a.warmup, repeat=not a.evolve)
         log(f"[bench]   {mixed_prev['elapsed'] * 1e3 / a.steps:.2f} ms/step")
 
+    mixed_pf = None
+    if a.face_path_ab:
+        log("[bench] mixed run with one thread per face (face_path = 1)")
+        mixed_pf = run_timed(sm, pr0, a.steps, a.warmup, mlp=mlp, repeat=not a.evolve, face_path=1)
+        log(f"[bench]   {mixed_pf['elapsed'] * 1e3 / a.steps:.2f} ms/step")
+
     value = zu / mixed["elapsed"]
     nb = 9 if s["PHYSICS"] == "rmhd" else 5
     rm = s["PHYSICS"] == "rmhd"
@@ -465,6 +475,10 @@ def main():
                                        "ms_per_step": round(mixed_prev["elapsed"] * 1e3 / a.steps, 3),
                                        "kernel_ms_per_step": kernel_ms(mixed_prev["kernels"], a.steps),
                                        "c2p": c2p_report(mixed_prev["stats"])}
+    if mixed_pf is not None:
+        res["mixed_per_face_kernels"] = {"value": round(zu / mixed_pf["elapsed"], 1),
+                                         "ms_per_step": round(mixed_pf["elapsed"] * 1e3 / a.steps, 3),
+                                         "kernel_ms_per_step": kernel_ms(mixed_pf["kernels"], a.steps)}
     if plain is not None:
         res["fp64_plain"] = {"value": round(zu / plain["elapsed"], 1),
                              "ms_per_step": round(plain["elapsed"] * 1e3 / a.steps, 3),
