@@ -219,3 +219,38 @@ def test_previous_state_guess_matches_plain():
     r = np.abs(outs[0][core] - outs[1][core]).max() / np.abs(outs[0][core]).max()
     log(f"  max rel diff {r:.3e}")
     assert r <= 1e-9
+
+
+@pytest.mark.parametrize("kind", ["zero", "nan"])
+def test_bad_network_falls_back_to_plain(kind):
+    """A useless network (all weights zero) or a broken one (all NaN) cannot change the result: the
+    acceptance test rejects every start it cannot confirm, those cells take the plain con2prim,
+    and the step equals the plain step.  With NaN weights no start is accepted."""
+    log(f"test_bad_network_falls_back_to_plain {kind}")
+    from tests.golden_states import smooth_state
+    ng = 3
+    pr = smooth_state((20 + 2 * ng, 18 + 2 * ng, 16 + 2 * ng), 79, "rmhd", vamp=0.5)
+    w = np.zeros(learned.NPARAM, dtype=np.float32)
+    if kind == "nan":
+        w[:] = np.nan
+    outs, stats = [], []
+    for guess in ("plain", "learned"):
+        d = make_domain("rmhd", pr.shape, ng, gamma=4.0 / 3.0, recon="weno", riemann="hlld_full", rk_order=3,
+                        c2p_guess=guess)
+        if guess == "learned":
+            d.set_mlp_weights(w)
+        d.set_instrumentation(stats=True)
+        d.set_prim(pr)
+        d.step(1e-3)
+        outs.append(d.get_prim())
+        stats.append(d.stats())
+        log(f"  {guess}: {stats[-1]}")
+        d.close()
+    core = (slice(None),) + (slice(ng, -ng),) * 3
+    a, b = outs[0][core], outs[1][core]
+    assert np.all(np.isfinite(b))
+    r = np.abs(a - b).max() / np.abs(a).max()
+    log(f"  max rel diff {r:.3e}")
+    assert r <= 1e-9
+    if kind == "nan":
+        assert stats[1]["learned_ok"] == 0
