@@ -136,6 +136,33 @@ def test_driver_matches_fixture(tmp_path, monkeypatch):
     assert any(l.startswith("[rank0] t=") for l in lines) and lines[-1] == "Done."
 
 
+@pytest.mark.parametrize("asserts", [False, True])
+def test_driver_nan_check(tmp_path, monkeypatch, asserts):
+    """solvers/srhd3d_mpi_muscl.py:337-343: with CHECK_NAN_EVERY the driver counts non-finite
+    primitives on the device, warns, and raises RuntimeError when ASSERTS is set.  A NaN is planted
+    in one interior cell of a restart checkpoint."""
+    log(f"test_driver_nan_check asserts={asserts}")
+    from astrofd_amd import driver, ic
+    s = finalize(dict(load_config_file(os.path.join(ROOT, "astrofd_amd", "config", "srhd_jet_32.json")),
+                      CHECK_NAN_EVERY=1, ASSERTS=asserts, MAX_STEPS=2, OUT_EVERY=1000, PRINT_EVERY=1000))
+    ic.apply_derived_ic_keys(s)
+    pr = ic.initial_state(s)
+    pr[0, 10, 12, 14] = np.nan
+    dx = s["Lx"] / s["NX"]
+    ckpt = tmp_path / "checkpoint_rank0000_step000000.npz"
+    np.savez(ckpt, prim=pr, meta=np.array([dx, dx, dx, 0.0, 0], dtype=np.float64))
+    s["RESTART_PATH"] = str(ckpt)
+    monkeypatch.chdir(tmp_path)
+    lines = []
+    if asserts:
+        with pytest.raises(RuntimeError, match="NaN/Inf"):
+            driver.run(s, lines.append)
+    else:
+        _, _, steps = driver.run(s, lines.append)
+        assert steps == 2
+    assert sum(l == "[warn] NaN/Inf detected in primitives!" for l in lines) == (1 if asserts else 2)
+
+
 @pytest.mark.parametrize("phys", ["hydro", "rmhd"])
 def test_diagnostics_vs_oracle(phys):
     """Device reductions of utils/diagnostics.py against the numpy oracle on seeded states."""
