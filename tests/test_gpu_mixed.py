@@ -57,6 +57,14 @@ def test_mixed_step_passives_vs_oracle(phys, recon, lim, riem):
     check_mixed_step(phys, recon, lim, riem, 3, (22, 20, 18), vamp=0.2, npass=2)
 
 
+@pytest.mark.parametrize("riem", ["hlle", "hlld"])
+def test_mixed_weno_other_riemann_solvers(riem):
+    """The shared-edge FP32 WENO5 RMHD faces with HLLE (the reference's WENO RMHD config) and hlld,
+    against the FP64 oracle with the bars above."""
+    log(f"test_mixed_weno_other_riemann_solvers {riem}")
+    check_mixed_step("rmhd", "weno", "mc", riem, 3, (22, 20, 18))
+
+
 @pytest.mark.parametrize("riem", ["hlld_full", "hlld"])
 def test_mixed_per_face_fp32_faces(riem):
     """afd_params.face_path = 1: the FP32 WENO5 RMHD faces with one thread per face, the path that
