@@ -57,6 +57,15 @@ def test_mixed_step_passives_vs_oracle(phys, recon, lim, riem):
     check_mixed_step(phys, recon, lim, riem, 3, (22, 20, 18), vamp=0.2, npass=2)
 
 
+@pytest.mark.parametrize("riem", ["hlle", "hllc"])
+@pytest.mark.parametrize("interior", [(19, 16, 70), (21, 18, 130), (2, 3, 2)])
+def test_mixed_srhd_weno_faces(interior, riem):
+    """The FP32 SRHD WENO5 faces (one thread per face: sharing cell edges gained 12 % on the z faces
+    only, 2 % of the 256^3 step) on odd extents, against the FP64 oracle with the bars above."""
+    log(f"test_mixed_srhd_weno_faces {interior} {riem}")
+    check_mixed_step("hydro", "weno", "mc", riem, 3, interior, vamp=0.2)
+
+
 @pytest.mark.parametrize("riem", ["hlle", "hlld"])
 def test_mixed_weno_other_riemann_solvers(riem):
     """The shared-edge FP32 WENO5 RMHD faces with HLLE (the reference's WENO RMHD config) and hlld,
