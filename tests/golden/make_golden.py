@@ -298,6 +298,38 @@ def gen_rhs():
           resistive=np.array([0] * len(RMHD_RHS_CASES) + [1] * len(extra)), dxyz=np.array([dx, dy, dz]), **arrs)
 
 
+class WrapPastEnd(np.ndarray):
+    """A view whose scalar reads past the end of an axis wrap to its start.  The reference's RMHD
+    MUSCL passive stencils read pr[idx, i+3] at i = nx-3 (and j+3, k+3 likewise), which is an
+    IndexError in pure Python and a read of the next row in Numba; the GPU library wraps such a
+    read.  Negative indices keep Python's own wrap."""
+
+    def __getitem__(self, key):
+        if isinstance(key, tuple) and all(isinstance(k, (int, np.integer)) for k in key):
+            key = tuple(int(k) % n if k >= n else k for k, n in zip(key, self.shape))
+        return np.ndarray.__getitem__(self, key)
+
+
+RMHD_PASSIVE_RHS_CASES = [("muscl", "mc", "hlle"), ("muscl", "vanleer", "hlld"), ("muscl", "minmod", "hlld_full")]
+
+
+def gen_rhs_passive():
+    """RMHD MUSCL right-hand sides with two passive scalars (core/rmhd_core.py:832-940 and the y/z
+    counterparts: every variant's fluxes, right faces with the left face's speeds)."""
+    shape = (11, 12, 13)
+    dx, dy, dz = 1.0 / 5, 1.0 / 6, 1.0 / 7
+    arrs = {}
+    for c, (recon, lim, riem) in enumerate(RMHD_PASSIVE_RHS_CASES):
+        rmhd_configure(RECON=recon, LIMITER=lim, RIEMANN=riem, N_PASSIVE=2, PASSIVE_OFFSET=9)
+        pr = smooth_state(shape, 600 + c, "rmhd", vamp=0.2, npass=2)
+        t0 = time.time()
+        rhs = rmhd_core.compute_rhs_rmhd(pr.view(WrapPastEnd), *shape, dx, dy, dz)
+        arrs[f"pr_{c}"] = pr; arrs[f"rhs_{c}"] = np.asarray(rhs)
+        print(f"[golden] rmhd rhs passive {recon}/{lim}/{riem} {time.time()-t0:.1f}s", flush=True)
+    _save("rmhd_rhs_passive", cases=np.array(["/".join(c) for c in RMHD_PASSIVE_RHS_CASES]),
+          dxyz=np.array([dx, dy, dz]), **arrs)
+
+
 SRHD_STEP_CASES = [  # recon, limiter, riemann, rk, npass
     ("muscl", "minmod", "hlle", 2, 0), ("muscl", "vanleer", "hllc", 3, 0),
     ("ppm", "mc", "hlle", 2, 0), ("weno", "mc", "hllc", 3, 0), ("muscl", "mc", "hlle", 2, 1),
@@ -748,7 +780,7 @@ def gen_driver_32():
     _save("driver_srhd_32", pr0=pr0, out=out, dts=dts, settings=np.array(repr(s)))
 
 
-GENS = dict(pointwise=gen_pointwise, rhs=gen_rhs, steps=gen_steps, bc=gen_bc, sources=gen_sources,
+GENS = dict(pointwise=gen_pointwise, rhs=gen_rhs, rhs_passive=gen_rhs_passive, steps=gen_steps, bc=gen_bc, sources=gen_sources,
             driver_small=gen_driver_small, driver_sources=gen_driver_sources,
             dissipation=gen_dissipation, driver_diss=gen_driver_dissipation, gr=gen_gr, driver_gr=gen_driver_gr)
 

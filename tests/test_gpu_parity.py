@@ -59,6 +59,25 @@ def test_rhs_golden(phys):
         assert err <= 1e-11, f"{phys} rhs {name}: {err}"
 
 
+def test_rmhd_muscl_passive_rhs_golden():
+    """RMHD MUSCL with two passives against the reference's right-hand sides (written with the
+    reference's read past the array end at i = nx-3 wrapped to 0, as the library does)."""
+    f = load("rmhd_rhs_passive")
+    dx, dy, dz = f["dxyz"]
+    for c, name in enumerate(f["cases"]):
+        recon, lim, riem = str(name).split("/")
+        pr = f[f"pr_{c}"]
+        log(f"[gpu] rhs rmhd passive {name}")
+        with make_domain("rmhd", pr.shape, 3, (dx, dy, dz), gamma=4.0 / 3.0, limiter=lim, recon=recon, riemann=riem,
+                         n_passive=2) as d:
+            d.set_prim(pr)
+            rhs = d.compute_rhs()
+        err = rel_err(rhs, f[f"rhs_{c}"])
+        errp = rel_err(rhs[9:], f[f"rhs_{c}"][9:])
+        log(f"      rel err {err:.3e}  passives {errp:.3e}")
+        assert err <= 1e-11 and errp <= 1e-11, f"rmhd passive rhs {name}: {err} {errp}"
+
+
 # ---------------------------------------------------------------- steps vs golden
 @pytest.mark.parametrize("phys", ["hydro", "rmhd"])
 def test_step_golden(phys):
@@ -231,4 +250,26 @@ def test_step_vs_oracle_24(phys, recon, lim, riem, rk):
         out = d.get_prim()
     err = per_var_rel(out, ref)
     log(f"      rel err {err:.3e}")
+    assert err <= 1e-10
+
+
+@pytest.mark.parametrize("riem,rk", [("hlld", 3), ("hlle", 2)])
+def test_rmhd_muscl_passive_step_vs_oracle(riem, rk):
+    """RMHD MUSCL SSPRK step with two passives (their fluxes on every face, right faces with the
+    left face's speeds) against the oracle, whose passive RHS is pinned to the reference."""
+    from tests.golden_states import smooth_state
+    shape = (11, 28, 26, 24)
+    pr = smooth_state(shape[1:], 81, "rmhd", vamp=0.2, npass=2)
+    dx = 1.0 / 22
+    P = O.make_params("rmhd", shape[1:], nv=11, ng=3, gamma=4.0 / 3.0, p_max=2.0, v_max=0.9999, limiter="vanleer",
+                      recon="muscl", riemann=riem, rk_order=rk, n_passive=2)
+    ref = O.step(P, pr, dx, dx, dx, 0.4 * dx)
+    log(f"[gpu] rmhd muscl passive step {riem} rk{rk}")
+    with make_domain("rmhd", shape, 3, (dx, dx, dx), gamma=4.0 / 3.0, limiter="vanleer", recon="muscl",
+                     riemann=riem, rk_order=rk, n_passive=2) as d:
+        d.set_prim(pr)
+        d.step(0.4 * dx)
+        out = d.get_prim()
+    err = per_var_rel(out, ref)
+    log(f"      rel err {err:.3e} passives {per_var_rel(out[9:], ref[9:]):.3e}")
     assert err <= 1e-10

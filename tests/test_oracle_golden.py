@@ -107,6 +107,23 @@ def test_rmhd_rhs():
         assert_bitwise(O.rhs(P, pr, dx, dy, dz), f[f"rhs_{c}"], f"rhs {name} resistive={res}")
 
 
+def test_rmhd_muscl_passive_rhs():
+    """RMHD MUSCL with two passives (core/rmhd_core.py:832-940 and the y/z counterparts).  The
+    reference reads pr[idx, i+3] past the end of the array at i = nx-3 (an IndexError in pure
+    Python, the next row in Numba); the fixture was written with that read wrapped to index 0,
+    the oracle's and the GPU's choice, so this pins every other term of the passive fluxes."""
+    f = load("rmhd_rhs_passive")
+    dx, dy, dz = f["dxyz"]
+    for c, name in enumerate(f["cases"]):
+        recon, lim, riem = str(name).split("/")
+        pr = f[f"pr_{c}"]
+        P = O.make_params("rmhd", pr.shape[1:], ng=3, gamma=4.0 / 3.0, p_max=2.0, v_max=0.9999, limiter=lim,
+                          recon=recon, riemann=riem, n_passive=2)
+        out = O.rhs(P, pr, dx, dy, dz)
+        assert np.abs(out[9:]).max() > 0.0
+        assert_bitwise(out, f[f"rhs_{c}"], f"rhs {name} with passives")
+
+
 def test_srhd_step():
     f = load("srhd_step")
     dx, dy, dz = f["dxyz"]; dt = float(f["dt"])
