@@ -122,6 +122,7 @@ SIGNATURES = {
     "afd_set_instrumentation": [_VP, _i32, _i32],
     "afd_get_stats": [_VP, ctypes.POINTER(Stats)],
     "afd_reset_stats": [_VP],
+    "afd_get_c2p_flags": [_VP, ctypes.POINTER(ctypes.c_uint8), _i64],
     "afd_synchronize": [_VP],
     "afd_mlp_param_count": [ctypes.POINTER(_i32)],
     "afd_mlp_set_weights": [_VP, ctypes.POINTER(_f32), _i32],

@@ -34,6 +34,7 @@ UNITS = [
     ("afd_common.hip", []),
     ("afd_srhd.hip", []),
     ("afd_mlp.hip", []),
+    ("afd_fused.hip", []),
     ("afd_rmhd.hip", ["-DAFD_RMHD_T=double"], "afd_rmhd_f64"),
     ("afd_rmhd.hip", ["-DAFD_RMHD_T=float"], "afd_rmhd_f32"),
 ]

@@ -1,8 +1,9 @@
 // afd_capi.hip -- the extern "C" entry points of libafd.so (declared in include/afd.h).
 //
 // A context owns the device-resident state of one domain: three primitive buffers
-// (step input and two stage outputs), the conserved states U0 and U_s, the face-flux
+// (step input and two stage outputs), the running conserved state U_s, the face-flux
 // arrays of the three directions, the nozzle profile and the reduction scratch.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -32,13 +33,14 @@ struct afd_ctx {
     Grid g{};
     int nb = 5;
     bool carried = false, passive_flux = false;
+    bool dmode = false;  // mixed RMHD WENO5 with shared edges: the faces accumulate the flux divergence in Fx
     int diss = 0;  // FaceArgs::diss of the SRHD faces
     double *Puse = nullptr;  // GR ORTHONORMAL_FLUX: the stage input with rescaled velocities
     size_t fsize = 8;
     hipStream_t stream = nullptr;
     double *P[3] = {nullptr, nullptr, nullptr};
     int cur = 0;
-    double *U0 = nullptr, *Uc = nullptr;
+    double *Uc = nullptr;
     void *Fx = nullptr, *Fxc = nullptr, *Fy = nullptr, *Fz = nullptr, *zs = nullptr;
     double *aux[3] = {nullptr, nullptr, nullptr};
     double *rhs_pass = nullptr, *rhs_out = nullptr;
@@ -57,6 +59,7 @@ struct afd_ctx {
     // instrumentation
     bool stats_on = false, timing_on = false;
     unsigned long long *stats_dev = nullptr;
+    unsigned char *flags_dev = nullptr;  // per-cell con2prim branch bits of the last stage (stats on)
     afd_stats acc{};
     struct Mark {
         int cat;
@@ -238,6 +241,8 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
     c->carried = nb == 9 && p.recon_id != AFD_RECON_MUSCL;
     c->passive_flux = p.n_passive > 0 && (nb == 9 ? p.recon_id == AFD_RECON_MUSCL : p.recon_id != AFD_RECON_WENO);
     c->fsize = p.precision == AFD_PREC_MIXED ? 4 : 8;
+    c->dmode = nb == 9 && p.precision == AFD_PREC_MIXED && p.recon_id == AFD_RECON_WENO && p.face_path == 0 &&
+               g.ncell < (1L << 29);
     if (nb == 5) {  // core/srhd_core.py:393-462: bulk pressure (MUSCL), dissipative fluxes (MUSCL 1x, PPM 2x)
         int reps = p.nv >= 15 ? (p.recon_id == AFD_RECON_MUSCL ? 1 : (p.recon_id == AFD_RECON_PPM ? 2 : 0)) : 0;
         c->diss = (p.recon_id == AFD_RECON_MUSCL && p.passive_offset > 5 ? 1 : 0) | (reps << 1);
@@ -254,13 +259,14 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
     size_t N = (size_t)g.ncell;
     for (int b = 0; b < 3; ++b) TRY(dalloc(c, (void **)&c->P[b], N * p.nv * 8));
     if (p.gr_orthonormal && (p.gr_metric == 1 || p.gr_metric == 2)) TRY(dalloc(c, (void **)&c->Puse, N * p.nv * 8));
-    TRY(dalloc(c, (void **)&c->U0, N * nb * 8));
     TRY(dalloc(c, (void **)&c->Uc, N * nb * 8));
-    TRY(dalloc(c, &c->Fx, N * nb * c->fsize));
-    TRY(dalloc(c, &c->Fy, N * nb * c->fsize));
-    TRY(dalloc(c, &c->Fz, N * nb * c->fsize));
+    TRY(dalloc(c, &c->Fx, N * nb * c->fsize));  // the flux-divergence accumulator in D-mode
+    if (!c->dmode) {
+        TRY(dalloc(c, &c->Fy, N * nb * c->fsize));
+        TRY(dalloc(c, &c->Fz, N * nb * c->fsize));
+    }
     if (c->carried) {
-        TRY(dalloc(c, &c->Fxc, N * nb * c->fsize));
+        if (!c->dmode) TRY(dalloc(c, &c->Fxc, N * nb * c->fsize));
         TRY(dalloc(c, &c->zs, N * 2 * c->fsize));
     }
     if (c->passive_flux) {
@@ -278,6 +284,11 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
     if (nb == 9 && p.c2p_guess != AFD_C2P_PLAIN) {
         long blocks = (long)grid1d(c->g.ncell).x;
         c->c2p_qcap = (blocks + QUEUE_SLOTS - 1) / QUEUE_SLOTS * BLOCK;
+        if (c->dmode) {  // k_face_march_update (up to 64 x MARCH_CELLS cells a workgroup) + k_update_shell
+            long fused = (fused_blocks(c->g) + QUEUE_SLOTS - 1) / QUEUE_SLOTS * 64L * MARCH_CELLS;
+            long shell = ((long)grid1d(shell_cells(c->g)).x + QUEUE_SLOTS - 1) / QUEUE_SLOTS * BLOCK;
+            c->c2p_qcap = std::max(c->c2p_qcap, fused + shell);
+        }
         TRY(dalloc(c, (void **)&c->c2p_queue, sizeof(int) * (size_t)c->c2p_qcap * QUEUE_SLOTS));
         TRY(dalloc(c, (void **)&c->c2p_qcount, sizeof(unsigned) * QUEUE_SLOTS));
     }
@@ -543,10 +554,12 @@ static FaceLaunch face_launch(afd_ctx *c, const double *P) {
     l.diss = c->diss;
     l.mixed = c->p.precision == AFD_PREC_MIXED;
     l.shared_edges = c->p.face_path == 0;
+    l.D = c->dmode ? (float *)c->Fx : nullptr;
+    l.idn = 0.0f;
     return l;
 }
 
-static int run_faces(afd_ctx *c, const double *P) {
+static int run_faces(afd_ctx *c, const double *P, bool skip_y = false) {
     if (c->Puse) {  // GR: the fluxes see the rescaled velocities (core/grhd_core.py:22-45)
         GrRescaleArgs r{P, c->Puse, c->g, c->p.ng, c->p.gr_x0, c->p.gr_mass, c->p.dx, c->p.dy, c->p.dz};
         Timed t(c, 4);
@@ -560,7 +573,9 @@ static int run_faces(afd_ctx *c, const double *P) {
     const int *order = c->carried ? order_carried : order_plain;
     for (int n = 0; n < 3; ++n) {
         int d = order[n];
+        if (skip_y && d == 1) continue;  // the stage update evaluates the y faces itself
         l.dir = d;
+        l.idn = (float)(1.0 / (d == 0 ? c->p.dx : (d == 1 ? c->p.dy : c->p.dz)));
         l.F = Fd[d];
         l.Fcar = (c->carried && d == 0) ? c->Fxc : nullptr;
         l.zs = c->carried ? c->zs : nullptr;
@@ -599,14 +614,15 @@ static UpdLaunch upd_launch(afd_ctx *c, const double *Pin, const double *P0, dou
     a.P0 = P0;
     a.Pin = Pin;
     a.Pout = Pout;
-    a.U0 = c->U0;
     a.Uc = c->Uc;
     a.Fx = (const double *)c->Fx;
     a.FxL = (const double *)(c->carried ? c->Fxc : c->Fx);
     a.Fy = (const double *)c->Fy;
     a.Fz = (const double *)c->Fz;
+    a.D = c->dmode ? (const float *)c->Fx : nullptr;
     a.rhs_pass = c->passive_flux ? c->rhs_pass : nullptr;
     a.stats = c->stats_on ? c->stats_dev : nullptr;
+    a.flags = c->stats_on ? c->flags_dev : nullptr;
     a.g = c->g;
     a.stage = stage;
     a.nst = nst;
@@ -648,11 +664,20 @@ extern "C" int afd_step(afd_ctx *c, double dt) {
     int src = in;
     for (int s = 0; s < nst; ++s) {
         int dst = (s % 2 == 0) ? a : b;
-        int rc = run_faces(c, c->P[src]);
+        int rc = run_faces(c, c->P[src], c->dmode);
         if (rc) return rc;
         UpdLaunch l = upd_launch(c, c->P[src], c->P[in], c->P[dst], s, nst, dt);
         Timed t(c, 3);
-        hipError_t e = launch_update(l, false, c->stream);
+        hipError_t e;
+        if (c->dmode) {  // the y faces with the stage update
+            FaceLaunch fl = face_launch(c, c->Puse ? c->Puse : c->P[src]);
+            fl.dir = 1;
+            fl.idn = (float)(1.0 / c->p.dy);
+            fl.zs = c->zs;
+            e = launch_stage_fused(fl, l, c->stream);
+        } else {
+            e = launch_update(l, false, c->stream);
+        }
         if (e != hipSuccess) return fail(AFD_ERR_HIP, std::string("update kernel: ") + hipGetErrorString(e));
         src = dst;
     }
@@ -929,6 +954,12 @@ extern "C" int afd_centerline(afd_ctx *c, double *out, int64_t n) {
 // ------------------------------------------------------------------ instrumentation
 extern "C" int afd_set_instrumentation(afd_ctx *c, int32_t stats, int32_t timing) {
     if (!c) return fail(AFD_ERR_INVALID, "null context");
+    if (stats && !c->flags_dev) {
+        HIPCHK(hipSetDevice(c->p.device));
+        ALLOC(c, c->flags_dev, (size_t)c->g.ncell);
+        HIPCHK(hipMemsetAsync(c->flags_dev, 0, (size_t)c->g.ncell, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
     c->stats_on = stats != 0;
     c->timing_on = timing != 0;
     return AFD_OK;
@@ -950,6 +981,16 @@ extern "C" int afd_get_stats(afd_ctx *c, afd_stats *out) {
     out->bisections = v[3];
     out->invalid = v[4];
     out->learned_ok = v[5];
+    return AFD_OK;
+}
+
+extern "C" int afd_get_c2p_flags(afd_ctx *c, uint8_t *out, int64_t n) {
+    if (!c || !out) return fail(AFD_ERR_INVALID, "afd_get_c2p_flags: null argument");
+    if (n != (int64_t)c->g.ncell) return fail(AFD_ERR_INVALID, "afd_get_c2p_flags: n must be the number of cells");
+    if (!c->flags_dev) return fail(AFD_ERR_STATE, "afd_get_c2p_flags: no step has run with the counters on");
+    HIPCHK(hipSetDevice(c->p.device));
+    HIPCHK(hipMemcpyAsync(out, c->flags_dev, (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return AFD_OK;
 }
 
@@ -1110,7 +1151,10 @@ extern "C" int afd_mlp_forward(afd_ctx *c, int64_t n, const float *X, float *Y) 
     hipError_t e = hipSuccess;
     if (!rc) {
         e = hipMemcpyAsync(dX, X, (size_t)n * 20, hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess) e = mlp_forward(c->mlp.w, dX, n, dY, c->stream);
+        if (e == hipSuccess) {
+            Timed tm(c, 4);  // the network alone, in ms_other when timing is on
+            e = mlp_forward(c->mlp.w, dX, n, dY, c->stream);
+        }
         if (e == hipSuccess) e = hipMemcpyAsync(Y, dY, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     }

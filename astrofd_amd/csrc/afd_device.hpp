@@ -125,13 +125,56 @@ template <typename T> AFD_D T weno5_right(T qp2, T qp1, T q0, T qm1, T qm2) {
     return weno5_left(qp2, qp1, q0, qm1, qm2);
 }
 
+// FP32 WENO5 of large deviations.  The nonlinear weights a_k / sum(a_j) with a_k = c_k / (eps + IS_k)^2
+// are invariant under q -> q / D when eps scales as eps / D^2 (IS_k scales as D^2), and every
+// candidate polynomial is linear in q.  In FP32, (eps + IS_k)^2 overflows once the deviations
+// exceed about 4e9 (IS ~ q^2, squared again), and the weights underflow well before that: every
+// a_k becomes 0 and the weight sum 0 * (1 / 0) = NaN, where FP64 still returns the edge value.
+// The kernels evaluate every variable of a face or cell as the reference does, track the smallest
+// weight sum, and only where it falls below WENO_SUM_MIN evaluate that lane again on q / max|q|
+// with the scaled eps (floored at 1e-18, so that 1 / eps^2 stays finite; on such stencils eps is
+// far below every nonzero indicator either way), scaling the result back.  The floor only
+// matters where the weights differ from FP64's by far less than FP32 rounding.
+constexpr float WENO_SUM_MIN = 1e-30f;
+
+// core/srhd_core.py:343-359 in FP32 with a given eps; s = the weight sum
+AFD_D float weno5_left_f32_eps(float qm2, float qm1, float q0, float qp1, float qp2, float eps, float &s) {
+    float IS0 = (13.0f / 12.0f) * sq(qm2 - 2.0f * qm1 + q0) + 0.25f * sq(qm2 - 4.0f * qm1 + 3.0f * q0);
+    float IS1 = (13.0f / 12.0f) * sq(qm1 - 2.0f * q0 + qp1) + 0.25f * sq(qm1 - qp1);
+    float IS2 = (13.0f / 12.0f) * sq(q0 - 2.0f * qp1 + qp2) + 0.25f * sq(3.0f * q0 - 4.0f * qp1 + qp2);
+    float a0 = fdiv(0.1f, sq(eps + IS0));
+    float a1 = fdiv(0.6f, sq(eps + IS1));
+    float a2 = fdiv(0.3f, sq(eps + IS2));
+    s = a0 + a1 + a2;
+    float w0 = fdiv(a0, s), w1 = fdiv(a1, s), w2 = fdiv(a2, s);
+    float p0 = (1.0f / 3.0f) * qm2 - (7.0f / 6.0f) * qm1 + (11.0f / 6.0f) * q0;
+    float p1 = (-1.0f / 6.0f) * qm1 + (5.0f / 6.0f) * q0 + (1.0f / 3.0f) * qp1;
+    float p2 = (1.0f / 3.0f) * q0 + (5.0f / 6.0f) * qp1 - (1.0f / 6.0f) * qp2;
+    return w0 * p0 + w1 * p1 + w2 * p2;
+}
+
+// The FP32 WENO5 edge with the rescaled evaluation where the weights underflow.
+AFD_D float weno5_left_f32_rescaled(float qm2, float qm1, float q0, float qp1, float qp2) {
+    float s;
+    float r = weno5_left_f32_eps(qm2, qm1, q0, qp1, qp2, 1e-12f, s);
+    if (!(s >= WENO_SUM_MIN)) {
+        const float D = fmaxf(fmaxf(fmaxf(fabsf(qm2), fabsf(qm1)), fmaxf(fabsf(q0), fabsf(qp1))), fabsf(qp2));
+        if (D > 0.0f) {
+            const float iD = 1.0f / D;
+            const float e = fmaxf(1e-12f * iD * iD, 1e-18f);
+            r = D * weno5_left_f32_eps(qm2 * iD, qm1 * iD, q0 * iD, qp1 * iD, qp2 * iD, e, s);
+        }
+    }
+    return r;
+}
+
 // Both WENO5 edge values of one cell, from the deviations of its four stencil neighbours from
 // the cell's own value (the cell's deviation is 0): hi = weno5_left (the cell's right edge),
 // lo = weno5_right (its left edge).  The mirror image has the same three smoothness indicators
 // in reverse order, so one set of indicators and reciprocals serves both edges, and each edge
 // is normalised once: sum(a_k p_k) / sum(a_k) equals sum(w_k p_k) with w_k = a_k / sum(a_k).
-AFD_D void weno5_cell_pair(float dm2, float dm1, float dp1, float dp2, float &lo, float &hi) {
-    const float eps = 1e-12f;
+// Returns the smaller of the two weight sums.
+AFD_D float weno5_cell_pair(float dm2, float dm1, float dp1, float dp2, float &lo, float &hi, float eps = 1e-12f) {
     const float IS0 = (13.0f / 12.0f) * sq(dm2 - 2.0f * dm1) + 0.25f * sq(dm2 - 4.0f * dm1);
     const float IS1 = (13.0f / 12.0f) * sq(dm1 + dp1) + 0.25f * sq(dm1 - dp1);
     const float IS2 = (13.0f / 12.0f) * sq(dp2 - 2.0f * dp1) + 0.25f * sq(dp2 - 4.0f * dp1);
@@ -143,13 +186,31 @@ AFD_D void weno5_cell_pair(float dm2, float dm1, float dp1, float dp2, float &lo
     const float h1 = (-1.0f / 6.0f) * dm1 + (1.0f / 3.0f) * dp1;
     const float h2 = (5.0f / 6.0f) * dp1 - (1.0f / 6.0f) * dp2;
     const float ha0 = 0.1f * r0, ha1 = 0.6f * r1, ha2 = 0.3f * r2;
-    hi = (ha0 * h0 + ha1 * h1 + ha2 * h2) * __builtin_amdgcn_rcpf(ha0 + ha1 + ha2);
+    const float sh = ha0 + ha1 + ha2;
+    hi = (ha0 * h0 + ha1 * h1 + ha2 * h2) * __builtin_amdgcn_rcpf(sh);
     // left edge: the mirror image, weights 0.1, 0.6, 0.3 on stencils (0..2), (-1..1), (-2..0)
     const float l0 = (1.0f / 3.0f) * dp2 - (7.0f / 6.0f) * dp1;
     const float l1 = (-1.0f / 6.0f) * dp1 + (1.0f / 3.0f) * dm1;
     const float l2 = (5.0f / 6.0f) * dm1 - (1.0f / 6.0f) * dm2;
     const float la0 = 0.1f * r2, la2 = 0.3f * r0;
-    lo = (la0 * l0 + ha1 * l1 + la2 * l2) * __builtin_amdgcn_rcpf(la0 + ha1 + la2);
+    const float sl = la0 + ha1 + la2;
+    lo = (la0 * l0 + ha1 * l1 + la2 * l2) * __builtin_amdgcn_rcpf(sl);
+    return fminf(sh, sl);
+}
+
+// weno5_cell_pair with the rescaled evaluation where the weights underflow.
+AFD_D void weno5_cell_pair_rescaled(float dm2, float dm1, float dp1, float dp2, float &lo, float &hi) {
+    const float s = weno5_cell_pair(dm2, dm1, dp1, dp2, lo, hi);
+    if (!(s >= WENO_SUM_MIN)) {
+        const float D = fmaxf(fmaxf(fabsf(dm2), fabsf(dm1)), fmaxf(fabsf(dp1), fabsf(dp2)));
+        if (D > 0.0f) {
+            const float iD = 1.0f / D;
+            const float e = fmaxf(1e-12f * iD * iD, 1e-18f);
+            weno5_cell_pair(dm2 * iD, dm1 * iD, dp1 * iD, dp2 * iD, lo, hi, e);
+            lo *= D;
+            hi *= D;
+        }
+    }
 }
 
 // core/srhd_core.py:171-186
@@ -757,7 +818,7 @@ AFD_D bool rmhd_c2p_from_guess(const RmhdC2PParams &c, const double *U, double Z
     return true;
 }
 
-// FP64 p2c of a cell state (stage-1 U0): core/srhd_core.py:46-59, core/rmhd_core.py:137-168
+// FP64 p2c of a cell state (U0 of the SSPRK stages): core/srhd_core.py:46-59, core/rmhd_core.py:137-168
 AFD_D void srhd_p2c_cell(double gfac, const double *q, double *U) {
     double v2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
     double W = lorentz_guarded(v2);

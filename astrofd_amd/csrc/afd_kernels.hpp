@@ -78,15 +78,20 @@ template <typename T> struct FaceArgs {
     int lim;
     int diss;          // SRHD dissipative hooks: bit 0 bulk pressure on the face pressures, bits 1-2 flux repeats
     UDiv dk, dn;       // face box: division by its z extent, then by its x (DIR 0) or y extent
+    float *D;          // flux-divergence kernels: sum over directions of (F_right - F_left) / d, nb x ncell
+    float idn;         //   1 / spacing along the face normal
 };
 
 // ------------------------------------------------------------------ reconstruction
 // Face states of the face between cells c-1 and c along stride s, variable base pointer b.
-// The mixed path reconstructs deviations from q[c] in FP32 and adds them back in FP64.
-template <int RECON, typename T>
-AFD_D void recon_face(const double *__restrict__ b, long c, long s, int lim, double &qL, double &qR) {
+// The mixed path reconstructs deviations from q[c] in FP32 and adds them back in FP64.  Returns
+// the smaller WENO5 weight sum of the FP32 path (1 otherwise); below WENO_SUM_MIN the caller
+// evaluates the face again with RESCUE set (the rescaled FP32 WENO5).
+template <int RECON, typename T, bool RESCUE = false>
+AFD_D float recon_face(const double *__restrict__ b, long c, long s, int lim, double &qL, double &qR) {
     constexpr bool mixed = sizeof(T) == 4;
     const double ref = mixed ? b[c] : 0.0;
+    float smin = 1.0f;
     T l, r;
     if (RECON == 0) {  // MUSCL: core/srhd_core.py:486-533
         T qm2 = (T)(b[c - 2 * s] - ref), qm1 = (T)(b[c - s] - ref), q0 = (T)(b[c] - ref), qp1 = (T)(b[c + s] - ref);
@@ -101,9 +106,17 @@ AFD_D void recon_face(const double *__restrict__ b, long c, long s, int lim, dou
             T a, bb;
             ppm_reconstruct<T>(qm3, qm2, qm1, q0, qp1, a, l);
             ppm_reconstruct<T>(qm2, qm1, q0, qp1, qp2, r, bb);
-        } else {  // WENO5: core/srhd_core.py:1183-1184
+        } else if constexpr (!mixed) {  // WENO5: core/srhd_core.py:1183-1184
             l = weno5_left<T>(qm3, qm2, qm1, q0, qp1);
             r = weno5_right<T>(qp2, qp1, q0, qm1, qm2);
+        } else if constexpr (RESCUE) {
+            l = weno5_left_f32_rescaled(qm3, qm2, qm1, q0, qp1);
+            r = weno5_left_f32_rescaled(qp2, qp1, q0, qm1, qm2);
+        } else {
+            float s1, s2;
+            l = weno5_left_f32_eps(qm3, qm2, qm1, q0, qp1, 1e-12f, s1);
+            r = weno5_left_f32_eps(qp2, qp1, q0, qm1, qm2, 1e-12f, s2);
+            smin = fminf(s1, s2);
         }
     }
     if (mixed) {
@@ -112,6 +125,21 @@ AFD_D void recon_face(const double *__restrict__ b, long c, long s, int lim, dou
     } else {
         qL = (double)l;
         qR = (double)r;
+    }
+    return smin;
+}
+
+// The face states of NV variables (fields at b + v * n); the FP32 WENO5 lanes whose weights
+// underflow evaluate again with the rescaled WENO5.
+template <int RECON, typename T, int NV>
+AFD_D void recon_face_all(const double *__restrict__ b, long n, long c, long s, int lim, double *qL, double *qR) {
+    float smin = 1.0f;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) smin = fminf(smin, recon_face<RECON, T>(b + v * n, c, s, lim, qL[v], qR[v]));
+    if constexpr (RECON == 2 && sizeof(T) == 4) {
+        if (__builtin_expect(!(smin >= WENO_SUM_MIN), 0)) {
+            for (int v = 0; v < NV; ++v) recon_face<RECON, T, true>(b + v * n, c, s, lim, qL[v], qR[v]);
+        }
     }
 }
 
@@ -195,8 +223,7 @@ __global__ __launch_bounds__(BLOCK) void k_face_srhd(FaceArgs<T> a) {
     if (!face_coords<DIR>(g, a.dk, a.dn, face_thread<DIR>(), i, j, k)) return;
     const long c = cidx(g, i, j, k), s = dstride<DIR>(g);
     double qLd[5], qRd[5];
-#pragma unroll
-    for (int v = 0; v < 5; ++v) recon_face<RECON, T>(a.P + v * g.ncell, c, s, a.lim, qLd[v], qRd[v]);
+    recon_face_all<RECON, T, 5>(a.P, g.ncell, c, s, a.lim, qLd, qRd);
     if (a.diss & 1) {  // add_pi_lr, core/srhd_core.py:394-399 (MUSCL faces, PASSIVE_OFFSET > 5)
         qLd[4] += a.P[5 * g.ncell + c - s];
         qRd[4] += a.P[5 * g.ncell + c];
@@ -296,8 +323,7 @@ AFD_D void rmhd_face_setup(const FaceArgs<T> &a, long c, RmhdFace<DIR, T> &f) {
     const Grid &g = a.g;
     const long s = dstride<DIR>(g);
     State9 L, R;
-#pragma unroll
-    for (int v = 0; v < 9; ++v) recon_face<RECON, T>(a.P + v * g.ncell, c, s, a.lim, L.q[v], R.q[v]);
+    recon_face_all<RECON, T, 9>(a.P, g.ncell, c, s, a.lim, L.q, R.q);
     rmhd_face_states<DIR, T>(a, L, R, f);
 }
 
@@ -385,13 +411,33 @@ void k_face_rmhd(FaceArgs<T> a) {
 AFD_D double ld32(const double *__restrict__ b, unsigned e) { return *(const double *)((const char *)b + e * 8u); }
 
 // Edge values of cell x of one variable (base b) along stride s (grids below 2^29 cells), as
-// deviations l, h from the cell's value v0; vm1 is the value of cell x - s.
-AFD_D void weno5_cell_edges(const double *__restrict__ b, unsigned x, unsigned s, double &vm1, double &v0, float &l,
-                            float &h) {
+// deviations l, h from the cell's value v0; vm1 is the value of cell x - s.  Returns the smaller
+// WENO5 weight sum (below WENO_SUM_MIN: weno5_cell_edges_rescaled gives the edges).
+AFD_D float weno5_cell_edges(const double *__restrict__ b, unsigned x, unsigned s, double &vm1, double &v0, float &l,
+                             float &h) {
     v0 = ld32(b, x);
     vm1 = ld32(b, x - s);
-    weno5_cell_pair((float)(ld32(b, x - 2 * s) - v0), (float)(vm1 - v0), (float)(ld32(b, x + s) - v0),
-                    (float)(ld32(b, x + 2 * s) - v0), l, h);
+    return weno5_cell_pair((float)(ld32(b, x - 2 * s) - v0), (float)(vm1 - v0), (float)(ld32(b, x + s) - v0),
+                           (float)(ld32(b, x + 2 * s) - v0), l, h);
+}
+
+AFD_D void weno5_cell_edges_rescaled(const double *__restrict__ b, unsigned x, unsigned s, float &l, float &h) {
+    const double v0 = ld32(b, x);
+    weno5_cell_pair_rescaled((float)(ld32(b, x - 2 * s) - v0), (float)(ld32(b, x - s) - v0),
+                             (float)(ld32(b, x + s) - v0), (float)(ld32(b, x + 2 * s) - v0), l, h);
+}
+
+// Edge values of the nine variables of cell x: L.q = the values of cell x - s, R.q = those of
+// cell x, lo / hi = the cell's left / right edge deviations.  Lanes whose WENO5 weights
+// underflow evaluate again with the rescaled WENO5.
+AFD_D void weno5_cell_edges9(const double *__restrict__ P, unsigned n, unsigned x, unsigned s, State9 &L, State9 &R,
+                             float *lo, float *hi) {
+    float smin = 1.0f;
+#pragma unroll
+    for (int v = 0; v < 9; ++v) smin = fminf(smin, weno5_cell_edges(P + v * n, x, s, L.q[v], R.q[v], lo[v], hi[v]));
+    if (__builtin_expect(!(smin >= WENO_SUM_MIN), 0)) {
+        for (int v = 0; v < 9; ++v) weno5_cell_edges_rescaled(P + v * n, x, s, lo[v], hi[v]);
+    }
 }
 
 // x and y faces: a workgroup of ROW_WAVES waves covers 64 consecutive z positions of
@@ -425,14 +471,12 @@ void k_face_rmhd_weno_rows(FaceArgs<float> a) {
     const unsigned s = (unsigned)dstride<DIR>(g);
     const unsigned c = (unsigned)(DIR == 0 ? cidx(g, nc, o, k) : cidx(g, o, nc, k));
     State9 L, R;
+    float lo[9], hi[9];
+    weno5_cell_edges9(a.P, (unsigned)g.ncell, c, s, L, R, lo, hi);
 #pragma unroll
     for (int v = 0; v < 9; ++v) {
-        double vm1, v0;
-        float l, h;
-        weno5_cell_edges(a.P + v * g.ncell, c, s, vm1, v0, l, h);
-        edge[w][v][lane] = h;
-        L.q[v] = vm1;
-        R.q[v] = v0 + (double)l;
+        edge[w][v][lane] = hi[v];
+        R.q[v] += (double)lo[v];
     }
     __syncthreads();
     if (w == 0 || !live) return;
@@ -462,18 +506,167 @@ void k_face_rmhd_weno_lanes(FaceArgs<float> a) {
     const int i = g.r0[0] + (int)ii, j = g.r0[1] + (int)(line - ii * ny), k = g.r0[2] - 1 + (int)kk;
     const unsigned c = (unsigned)cidx(g, i, j, k);
     State9 L, R;
+    float lo[9], hi[9];
+    weno5_cell_edges9(a.P, (unsigned)g.ncell, c, 1u, L, R, lo, hi);
 #pragma unroll
-    for (int v = 0; v < 9; ++v) {
-        double vm1, v0;
-        float l, h;
-        weno5_cell_edges(a.P + v * g.ncell, c, 1u, vm1, v0, l, h);
-        L.q[v] = vm1 + (double)__shfl_up(h, 1);
-        R.q[v] = v0 + (double)l;
+    for (int v = 0; v < 9; ++v) {  // every lane takes part in the shuffles
+        L.q[v] += (double)__shfl_up(hi[v], 1);
+        R.q[v] += (double)lo[v];
     }
     if (!valid || lane == 0 || kk == 0) return;
     RmhdFace<2, float> fc;
     rmhd_face_states<2, float>(a, L, R, fc);
     rmhd_face_store<2, RIEM, 2, float>(a, fc, i, j, k, c);
+}
+
+// ------------------------------------------------------------------ RMHD WENO5 flux divergence, FP32 path
+// The mixed path's shared-edge WENO5 kernels accumulate each cell's flux divergence
+// D = sum_d (F(right face) - F(left face)) / d_d in FP32 (nb x ncell) instead of storing face
+// fluxes: the z kernel writes D, then the x and y kernels add their parts, and the stage update
+// reads one FP32 array (36 B a cell) where it read four flux arrays (144 B).  The z kernel still
+// stores the z-back speeds the x kernel's carried fluxes need.
+
+// The Riemann flux(es) of x or y face (i, j, k) (between cells c - s and c): Fown with the face's
+// own speeds (the right face of cell c - s; wanted when own), Fcar with the speeds carried from
+// the z-back face of the predecessor of cell c in the reference's i->j->k loop (x only; the left
+// face of cell c; wanted when car), as rmhd_face_store computes them.
+template <int RIEM, int DIR>
+AFD_D void rmhd_face_pair(const FaceArgs<float> &a, const RmhdFace<DIR, float> &f, int i, int j, int k, long c,
+                          bool own, bool car, float *Fown, float *Fcar) {
+    const Grid &g = a.g;
+    if (own) rmhd_riemann<2, RIEM, DIR, float>(a.ph, f, f.sL, f.sR, Fown);
+    if (DIR == 0 && car) {  // reached for DIR 0 only
+        float cL = -1.0f, cR = 1.0f;  // hlle_bounds_fast, core/rmhd_core.py:397-400
+        long pc = -1;
+        if (k > g.r0[2]) pc = c - 1;
+        else if (j > g.r0[1]) pc = cidx(g, i, j - 1, g.r1[2] - 1);
+        else if (i > g.r0[0]) pc = cidx(g, i - 1, g.r1[1] - 1, g.r1[2] - 1);
+        if (pc >= 0) {
+            cL = a.zs[pc];
+            cR = a.zs[g.ncell + pc];
+        }
+        rmhd_riemann<2, RIEM, DIR, float>(a.ph, f, cL, cR, Fcar);
+    }
+}
+
+// z faces: as k_face_rmhd_weno_lanes, with waves overlapping by two positions (wave w covers line
+// positions 62 w .. 62 w + 63): lane l's cell takes its right face from lane l + 1, so lanes
+// 1..62 write D for their cells (k = r0 .. r1 - 1 of a line) and the z-back speeds of their faces.
+// Every lane evaluates a face (lanes without one use in-range duplicates and write nothing), so
+// that every lane takes part in the shuffles.
+template <int RIEM>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(FaceOcc<float>::waves)))
+void k_face_rmhd_weno_lanes_div(FaceArgs<float> a) {
+    const Grid &g = a.g;
+    const int ncz = g.r1[2] - g.r0[2] + 2, ny = g.r1[1] - g.r0[1];
+    const unsigned npos = (unsigned)ncz * ny * (g.r1[0] - g.r0[0]);
+    const unsigned lane = threadIdx.x & 63u, w = (blockIdx.x * BLOCK + threadIdx.x) >> 6;
+    if (62u * w + 1u >= npos) return;  // whole waves past the last cell
+    unsigned pos = 62u * w + lane;
+    const bool valid = pos < npos;
+    if (!valid) pos = npos - 1;  // keeps the lane's loads in range; it writes nothing
+    const unsigned line = udiv(a.dk, pos), kk = pos - line * ncz, ii = udiv(a.dn, line);
+    const int i = g.r0[0] + (int)ii, j = g.r0[1] + (int)(line - ii * ny), k = g.r0[2] - 1 + (int)kk;
+    const unsigned c = (unsigned)cidx(g, i, j, k), n = (unsigned)g.ncell;
+    State9 L, R;
+    float lo[9], hi[9];
+    weno5_cell_edges9(a.P, n, c, 1u, L, R, lo, hi);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) {
+        L.q[v] += (double)__shfl_up(hi[v], 1);
+        R.q[v] += (double)lo[v];
+    }
+    RmhdFace<2, float> fc;
+    rmhd_face_states<2, float>(a, L, R, fc);
+    float F[9];
+    rmhd_riemann<2, RIEM, 2, float>(a.ph, fc, fc.sL, fc.sR, F);
+    float Fr[9];
+#pragma unroll
+    for (int v = 0; v < 9; ++v) Fr[v] = __shfl_down(F[v], 1);
+    if (!valid || lane == 0u || lane == 63u || kk == 0u) return;
+    a.zs[c] = fc.sL;
+    a.zs[n + c] = fc.sR;
+    if (kk + 2u > (unsigned)ncz) return;  // the face k = r1 closes a line: no cell of its own
+#pragma unroll
+    for (int v = 0; v < 9; ++v) a.D[v * n + c] = (Fr[v] - F[v]) * a.idn;
+}
+
+// x and y faces as pencils: a workgroup of four waves covers 64 consecutive z positions of one
+// transverse line and sweeps a run of MARCH_CELLS cells m0 .. m1 - 1 along the normal.  Round r:
+// wave w evaluates the edges of cell e = m0 - 1 + 4 r + w and the face (e - 1 | e), taking the
+// left cell's right edge from the previous wave through LDS (wave 0 from wave 3 of the round
+// before), then adds the face difference of cell e - 1 to D, taking the left face's flux from the
+// previous wave the same way.  A run evaluates the edges of m1 - m0 + 2 cells and m1 - m0 + 1
+// faces (its first face again at the next run's start); the LDS rows are double-buffered across
+// rounds.  Workgroups are ordered z chunk fastest, then run, then the transverse line, and walk
+// XCD-contiguous chunks.  a.dk divides by the number of z chunks, a.dn by the runs per line.
+#ifndef AFD_MARCH_CELLS
+#define AFD_MARCH_CELLS 30
+#endif
+constexpr int MARCH_CELLS = AFD_MARCH_CELLS;
+
+template <int RIEM, int DIR>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FaceOcc<float>::waves)))
+void k_face_rmhd_weno_march(FaceArgs<float> a) {
+    static_assert(DIR == 0 || DIR == 1, "pencils along x or y");
+    constexpr int O = DIR == 0 ? 1 : 0;  // the transverse axis other than z
+    __shared__ float edge[2][4][9][64];
+    __shared__ float flx[2][4][9][64];
+    const Grid &g = a.g;
+    const int n2 = g.r1[2] - g.r0[2], nkc = (n2 + 63) / 64;
+    const int nruns = (g.r1[DIR] - g.r0[DIR] + MARCH_CELLS - 1) / MARCH_CELLS;
+    const unsigned b = (unsigned)xcd_chunked_block(blockIdx.x, gridDim.x);
+    const unsigned r = udiv(a.dk, b), q = udiv(a.dn, r);
+    const int kc = (int)(b - r * nkc), run = (int)(r - q * nruns), o = g.r0[O] + (int)q;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int kraw = kc * 64 + lane;
+    const bool lane_ok = kraw < n2;
+    const int k = g.r0[2] + min(kraw, n2 - 1);  // in range for the loads
+    const int m0 = g.r0[DIR] + run * MARCH_CELLS, m1 = min(m0 + MARCH_CELLS, g.r1[DIR]);
+    const int count = m1 - m0 + 2;  // edge cells m0 - 1 .. m1
+    const int rounds = (count + 3) / 4;
+    const unsigned s = (unsigned)dstride<DIR>(g), n = (unsigned)g.ncell;
+    for (int rd = 0; rd < rounds; ++rd) {
+        const int buf = rd & 1, eidx = 4 * rd + w;
+        const int e = m0 - 1 + min(eidx, count - 1);  // waves past the run repeat its last cell
+        const unsigned c = (unsigned)(DIR == 0 ? cidx(g, e, o, k) : cidx(g, o, e, k));
+        State9 L, R;
+        float lo[9], hi[9];
+        weno5_cell_edges9(a.P, n, c, s, L, R, lo, hi);
+#pragma unroll
+        for (int v = 0; v < 9; ++v) edge[buf][w][v][lane] = hi[v];
+        __syncthreads();
+        const bool first = rd == 0 && w == 0;  // the halo cell m0 - 1: no face
+#pragma unroll
+        for (int v = 0; v < 9; ++v) {
+            const float hp = first ? 0.0f : (w > 0 ? edge[buf][w - 1][v][lane] : edge[buf ^ 1][3][v][lane]);
+            L.q[v] += (double)hp;
+            R.q[v] += (double)lo[v];
+        }
+        RmhdFace<DIR, float> fc;
+        rmhd_face_states<DIR, float>(a, L, R, fc);
+        // the face e is the right face of cell e - 1 (eidx >= 2) and the left face of cell e (eidx <= count - 2)
+        const bool right_of = eidx >= 2 && eidx < count, left_of = eidx >= 1 && eidx <= count - 2;
+        float Fown[9], Fex[9];
+        if constexpr (DIR == 0) {
+            rmhd_face_pair<RIEM, 0>(a, fc, e, o, k, (long)c, right_of, left_of, Fown, Fex);
+        } else {
+            rmhd_face_pair<RIEM, 1>(a, fc, o, e, k, (long)c, right_of || left_of, false, Fown, Fex);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) Fex[v] = Fown[v];
+        }
+#pragma unroll
+        for (int v = 0; v < 9; ++v) flx[buf][w][v][lane] = Fex[v];
+        __syncthreads();
+        if (lane_ok && right_of) {  // cell e - 1: right face e (Fown), left face e - 1 (the previous wave's)
+            const unsigned cl = c - s;
+#pragma unroll
+            for (int v = 0; v < 9; ++v) {
+                const float fl = w > 0 ? flx[buf][w - 1][v][lane] : flx[buf ^ 1][3][v][lane];
+                a.D[v * n + cl] += (Fown[v] - fl) * a.idn;
+            }
+        }
+    }
 }
 
 // ------------------------------------------------------------------ passive scalars (Q15)
@@ -528,13 +721,15 @@ __global__ void k_passive_rhs(PassiveArgs a);
 
 // ------------------------------------------------------------------ stage update + c2p
 template <typename F> struct UpdArgs {
-    const double *P0;     // step input primitives (U0 at stage 0, passives t0)
+    const double *P0;     // step input primitives (U0 = p2c(P0) at every stage, passives t0)
     const double *Pin;    // stage input primitives (GLM, resistivity, passives t_{s-1})
     double *Pout;         // stage output primitives
-    double *U0, *Uc;      // conserved: step start, running stage value (FP64)
+    double *Uc;           // conserved: running stage value (FP64); the step start's is p2c(P0)
     const F *Fx, *FxL, *Fy, *Fz;  // face fluxes; FxL is the left-face x flux (carried speeds)
+    const float *D;               // or (mixed RMHD WENO5 with shared edges) the FP32 flux divergence
     const double *rhs_pass;       // passive rhs (or null = 0)
     unsigned long long *stats;    // [hydro_iters, newton_evals, fallback, bisection, invalid, learned] or null
+    unsigned char *flags;         // with stats: the last stage's con2prim branch bits per cell (C2PStats::flags)
     Grid g;
     int stage, nst, n_resist, np, po;
     double dt, dx, dy, dz, glm_cp2, resistivity;
@@ -660,28 +855,11 @@ template <int NB, typename F> AFD_D void gr_source(const UpdArgs<F> &a, long c, 
     rhs[4] += U[1] * d[0] + U[2] * d[1] + U[3] * d[2];
 }
 
-// The flat RHS of a cell in the flux ranges: flux divergence, GLM and resistive sources.
-template <int NB, typename F> AFD_D void cell_rhs_flat(const UpdArgs<F> &a, long c, double *rhs) {
+// GLM damping and the resistive Laplacian of B (core/rmhd_core.py:1264-1275), n_resist times,
+// after the flux divergence.
+template <int NB, typename F> AFD_D void cell_rhs_glm_resist(const UpdArgs<F> &a, long c, double *rhs) {
 #pragma clang fp contract(off)
     const Grid &g = a.g;
-    if constexpr (sizeof(F) == 8) {  // FP64 path: the reference's divisions
-#pragma unroll
-        for (int v = 0; v < NB; ++v) {
-            long o = v * g.ncell + c;
-            rhs[v] -= ((double)a.Fx[o + g.sx] - (double)a.FxL[o]) / a.dx;
-            rhs[v] -= ((double)a.Fy[o + g.sy] - (double)a.Fy[o]) / a.dy;
-            rhs[v] -= ((double)a.Fz[o + 1] - (double)a.Fz[o]) / a.dz;
-        }
-    } else {  // mixed path (FP32 fluxes): multiply by the inverse spacings
-        const double ix = 1.0 / a.dx, iy = 1.0 / a.dy, iz = 1.0 / a.dz;
-#pragma unroll
-        for (int v = 0; v < NB; ++v) {
-            long o = v * g.ncell + c;
-            rhs[v] -= ((double)a.Fx[o + g.sx] - (double)a.FxL[o]) * ix;
-            rhs[v] -= ((double)a.Fy[o + g.sy] - (double)a.Fy[o]) * iy;
-            rhs[v] -= ((double)a.Fz[o + 1] - (double)a.Fz[o]) * iz;
-        }
-    }
     if (NB == 9) {
         rhs[8 % NB] -= a.glm_cp2 * a.Pin[8 * g.ncell + c];
         double invdx2 = 1.0 / (a.dx * a.dx + SMALL), invdy2 = 1.0 / (a.dy * a.dy + SMALL),
@@ -697,6 +875,34 @@ template <int NB, typename F> AFD_D void cell_rhs_flat(const UpdArgs<F> &a, long
             }
         }
     }
+}
+
+// The flat RHS of a cell in the flux ranges: flux divergence, GLM and resistive sources.
+template <int NB, typename F> AFD_D void cell_rhs_flat(const UpdArgs<F> &a, long c, double *rhs) {
+#pragma clang fp contract(off)
+    const Grid &g = a.g;
+    if constexpr (sizeof(F) == 8) {  // FP64 path: the reference's divisions
+#pragma unroll
+        for (int v = 0; v < NB; ++v) {
+            long o = v * g.ncell + c;
+            rhs[v] -= ((double)a.Fx[o + g.sx] - (double)a.FxL[o]) / a.dx;
+            rhs[v] -= ((double)a.Fy[o + g.sy] - (double)a.Fy[o]) / a.dy;
+            rhs[v] -= ((double)a.Fz[o + 1] - (double)a.Fz[o]) / a.dz;
+        }
+    } else if (a.D) {  // the face kernels accumulated the divergence, already over the spacings
+#pragma unroll
+        for (int v = 0; v < NB; ++v) rhs[v] -= (double)a.D[v * g.ncell + c];
+    } else {  // mixed path (FP32 fluxes): multiply by the inverse spacings
+        const double ix = 1.0 / a.dx, iy = 1.0 / a.dy, iz = 1.0 / a.dz;
+#pragma unroll
+        for (int v = 0; v < NB; ++v) {
+            long o = v * g.ncell + c;
+            rhs[v] -= ((double)a.Fx[o + g.sx] - (double)a.FxL[o]) * ix;
+            rhs[v] -= ((double)a.Fy[o + g.sy] - (double)a.Fy[o]) * iy;
+            rhs[v] -= ((double)a.Fz[o + 1] - (double)a.Fz[o]) * iz;
+        }
+    }
+    cell_rhs_glm_resist<NB, F>(a, c, rhs);
 }
 
 // The RHS of one cell: the flat RHS in the flux ranges (`in`), plus the GR sources on interior cells.
@@ -781,31 +987,26 @@ AFD_D bool cell_c2p(const UpdArgs<F> &a, const double *U, bool live, long c, dou
 // with the tiles interleaved; 5 waves spills and is 25 % slower.  The plain one is held to 5
 // (<= 96 VGPRs, no spills; 100 registers and 4 waves otherwise): 6 % faster in FP32, 2.4 % in
 // FP64 on that step (6 waves spills and gains nothing more).
+// The stage update of cell c (a lane with live set) from its right-hand side: the SSPRK
+// combination into Uc (FP64), the passives on primitives and the con2prim of the new state into
+// Pout.  Reached by every lane of the wave (the learned start runs wave-wide).
 template <int NB, typename F, bool LEARNED>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LEARNED ? 4 : 5))) void k_update(
-    UpdArgs<F> a) {
+AFD_D void cell_update(const UpdArgs<F> &a, long c, bool live, const double *rhs) {
     const Grid &g = a.g;
-    const long c = (long)blockIdx.x * BLOCK + threadIdx.x;
-    const bool live = c < g.ncell;
     C2PStats st = {0, 0, 0};
     double U[NB], q[NB];
 #pragma unroll
     for (int v = 0; v < NB; ++v) U[v] = 0.0;
+    const bool last = a.stage == a.nst - 1;
     if (live) {
-        double rhs[NB];
-        cell_rhs<NB, F>(a, c, in_range(g, c), rhs);
-        if (a.stage == 0) {
-            double q0[NB];
+        // U0 = p2c(P0) at every stage: the same arithmetic on the same input as at stage 0, so
+        // recomputing it costs the read of P0 that storing and re-reading U0 would cost, and
+        // saves the store
+        double q0[NB];
 #pragma unroll
-            for (int v = 0; v < NB; ++v) q0[v] = a.P0[v * g.ncell + c];
-            if constexpr (NB == 9) rmhd_p2c_cell(a.gfac, a.p_max, a.v_max, q0, U);
-            else srhd_p2c_cell(a.gfac, q0, U);
-#pragma unroll
-            for (int v = 0; v < NB; ++v) a.U0[v * g.ncell + c] = U[v];
-        } else {
-#pragma unroll
-            for (int v = 0; v < NB; ++v) U[v] = a.U0[v * g.ncell + c];
-        }
+        for (int v = 0; v < NB; ++v) q0[v] = a.P0[v * g.ncell + c];
+        if constexpr (NB == 9) rmhd_p2c_cell(a.gfac, a.p_max, a.v_max, q0, U);
+        else srhd_p2c_cell(a.gfac, q0, U);
 #pragma unroll
         for (int v = 0; v < NB; ++v) {
             double r;
@@ -815,7 +1016,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LEARNED ?
             }
             double uc = (a.stage == 0) ? 0.0 : a.Uc[v * g.ncell + c];
             U[v] = ssprk_combine(a.stage, a.nst, U[v], uc, r);
-            a.Uc[v * g.ncell + c] = U[v];
+            if (!last) a.Uc[v * g.ncell + c] = U[v];  // the last stage's U is read only by k_c2p_queue
         }
     }
     const bool have = cell_c2p<NB, LEARNED>(a, U, live, c, q, st);
@@ -823,7 +1024,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LEARNED ?
         if (have) {
 #pragma unroll
             for (int v = 0; v < NB; ++v) a.Pout[v * g.ncell + c] = q[v];
+        } else if (last) {  // queued for k_c2p_queue, which solves from Uc
+#pragma unroll
+            for (int v = 0; v < NB; ++v) a.Uc[v * g.ncell + c] = U[v];
         }
+        if (last && a.flags) a.flags[c] = (unsigned char)st.flags;
         // SRHD fields between the base variables and the passives (the dissipative Pi, pi^ij,
         // q^i) pass through the stage unchanged: core/srhd_core.py:1308-1309, 1326-1327
         for (int v = NB; v < a.po; ++v) a.Pout[(long)v * g.ncell + c] = a.Pin[(long)v * g.ncell + c];
@@ -841,6 +1046,138 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LEARNED ?
     if (a.stats) wave_add_stats(a.stats, st);
 }
 
+// Stage update of one cell per thread: flux divergence + sources, then cell_update.  LEARNED
+// selects the network's Newton start; a separate instantiation keeps its registers out of the
+// plain kernel.  The learned instantiation is held to 4 waves per SIMD (<= 128 VGPRs; it takes
+// 121, no spills, with the network's two tiles evaluated one after the other): on the 256^3 RMHD
+// WENO5 + HLLD step it ran 16 % slower at the compiler's own 2 waves, and 4 % slower at 3 waves
+// with the tiles interleaved; 5 waves spills and is 25 % slower.  The plain one is held to 5
+// (<= 96 VGPRs, no spills; 100 registers and 4 waves otherwise): 6 % faster in FP32, 2.4 % in
+// FP64 on that step (6 waves spills and gains nothing more).
+template <int NB, typename F, bool LEARNED>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LEARNED ? 4 : 5))) void k_update(
+    UpdArgs<F> a) {
+    const Grid &g = a.g;
+    const long c = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const bool live = c < g.ncell;
+    double rhs[NB];
+    if (live) cell_rhs<NB, F>(a, c, in_range(g, c), rhs);
+    cell_update<NB, F, LEARNED>(a, c, live, rhs);
+}
+
+// y pencils fused with the stage update (mixed RMHD WENO5 on the flux-divergence path): the pencil
+// of k_face_rmhd_weno_march<RIEM, 1> gives cell e - 1 its y difference, which is added to the z and
+// x parts in D as that kernel would store it, and the same wave updates the cell at once
+// (cell_update), so that the update's loads and stores overlap the other waves' face arithmetic.
+// The result is the one of the separate y kernel and k_update.  Cells outside the flux range are
+// updated by k_update_shell.
+template <int RIEM, bool LEARNED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FaceOcc<float>::waves)))
+void k_face_march_update(FaceArgs<float> a, UpdArgs<float> u) {
+    constexpr int DIR = 1, O = 0;
+    __shared__ float edge[2][4][9][64];
+    __shared__ float flx[2][4][9][64];
+    const Grid &g = a.g;
+    const int n2 = g.r1[2] - g.r0[2], nkc = (n2 + 63) / 64;
+    const int nruns = (g.r1[DIR] - g.r0[DIR] + MARCH_CELLS - 1) / MARCH_CELLS;
+    const unsigned b = (unsigned)xcd_chunked_block(blockIdx.x, gridDim.x);
+    const unsigned r = udiv(a.dk, b), q = udiv(a.dn, r);
+    const int kc = (int)(b - r * nkc), run = (int)(r - q * nruns), o = g.r0[O] + (int)q;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int kraw = kc * 64 + lane;
+    const bool lane_ok = kraw < n2;
+    const int k = g.r0[2] + min(kraw, n2 - 1);  // in range for the loads
+    const int m0 = g.r0[DIR] + run * MARCH_CELLS, m1 = min(m0 + MARCH_CELLS, g.r1[DIR]);
+    const int count = m1 - m0 + 2;  // edge cells m0 - 1 .. m1
+    const int rounds = (count + 3) / 4;
+    const unsigned s = (unsigned)dstride<DIR>(g), n = (unsigned)g.ncell;
+    for (int rd = 0; rd < rounds; ++rd) {
+        const int buf = rd & 1, eidx = 4 * rd + w;
+        const int e = m0 - 1 + min(eidx, count - 1);  // waves past the run repeat its last cell
+        const unsigned c = (unsigned)cidx(g, o, e, k);
+        const bool right_of = eidx >= 2 && eidx < count, left_of = eidx >= 1 && eidx <= count - 2;
+        float Fown[9];
+        {
+            State9 L, R;
+            float lo[9], hi[9];
+            weno5_cell_edges9(a.P, n, c, s, L, R, lo, hi);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) edge[buf][w][v][lane] = hi[v];
+            __syncthreads();
+            const bool first = rd == 0 && w == 0;  // the halo cell m0 - 1: no face
+#pragma unroll
+            for (int v = 0; v < 9; ++v) {
+                const float hp = first ? 0.0f : (w > 0 ? edge[buf][w - 1][v][lane] : edge[buf ^ 1][3][v][lane]);
+                L.q[v] += (double)hp;
+                R.q[v] += (double)lo[v];
+            }
+            RmhdFace<DIR, float> fc;
+            rmhd_face_states<DIR, float>(a, L, R, fc);
+            float unused[9];
+            rmhd_face_pair<RIEM, 1>(a, fc, o, e, k, (long)c, right_of || left_of, false, Fown, unused);
+        }
+#pragma unroll
+        for (int v = 0; v < 9; ++v) flx[buf][w][v][lane] = Fown[v];
+        __syncthreads();
+        // keep the face arithmetic and the update apart: interleaved, their live ranges overlap
+        __builtin_amdgcn_sched_barrier(0);
+        if (right_of) {  // wave-uniform: cell e - 1, right face e (Fown), left face e - 1 (the previous wave's)
+            const unsigned cl = c - s;
+            double rhs[9];
+            if (lane_ok) {
+#pragma unroll
+                for (int v = 0; v < 9; ++v) {
+                    const float fl = w > 0 ? flx[buf][w - 1][v][lane] : flx[buf ^ 1][3][v][lane];
+                    const float d = a.D[v * n + cl] + (Fown[v] - fl) * a.idn;
+                    rhs[v] = 0.0;
+                    rhs[v] -= (double)d;
+                }
+                cell_rhs_glm_resist<9, float>(u, (long)cl, rhs);
+                if (u.gr) gr_source<9, float>(u, (long)cl, rhs);
+            }
+            cell_update<9, float, LEARNED>(u, (long)cl, lane_ok, rhs);
+        }
+    }
+}
+
+// The stage update of the cells outside the flux range [r0, r1) per axis (their flux divergence is
+// zero; the reference's p2c -> c2p runs over the whole array), one thread per such cell: the x
+// planes below r0 and from r1 (all j, k), then the rows of in-range x below r0 and from r1 in y (all
+// k), then the in-range x, y lines' k below r0 and from r1.
+template <int NB, typename F, bool LEARNED>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LEARNED ? 4 : 5))) void k_update_shell(
+    UpdArgs<F> a) {
+    const Grid &g = a.g;
+    const long ni = g.r1[0] - g.r0[0], nj = g.r1[1] - g.r0[1];
+    const long ox = g.nx - ni, oy = g.ny - nj, oz = g.nz - (g.r1[2] - g.r0[2]);
+    const long nA = ox * g.ny * g.nz, nB = ni * oy * g.nz, nC = ni * nj * oz;
+    long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const bool live = t < nA + nB + nC;
+    long i = 0, j = 0, k = 0;
+    if (live) {
+        auto out = [&](long p, int ax) { return p < g.r0[ax] ? p : g.r1[ax] + (p - g.r0[ax]); };
+        if (t < nA) {
+            k = t % g.nz;
+            j = (t / g.nz) % g.ny;
+            i = out(t / ((long)g.nz * g.ny), 0);
+        } else if (t < nA + nB) {
+            t -= nA;
+            k = t % g.nz;
+            j = out((t / g.nz) % oy, 1);
+            i = g.r0[0] + t / ((long)g.nz * oy);
+        } else {
+            t -= nA + nB;
+            k = out(t % oz, 2);
+            j = g.r0[1] + (t / oz) % nj;
+            i = g.r0[0] + t / (oz * nj);
+        }
+    }
+    const long c = live ? cidx(g, (int)i, (int)j, (int)k) : 0;
+    double rhs[NB];
+    if (live) cell_rhs<NB, F>(a, c, false, rhs);
+    cell_update<NB, F, LEARNED>(a, c, live, rhs);
+}
+
 // Plain RMHD con2prim of the queued cells (rejected learned starts), densely packed:
 // blockIdx.y = queue slot, grid-stride over the slot's entries.
 template <typename F> __global__ __launch_bounds__(BLOCK) void k_c2p_queue(UpdArgs<F> a) {
@@ -855,9 +1192,14 @@ template <typename F> __global__ __launch_bounds__(BLOCK) void k_c2p_queue(UpdAr
         double U[9], q[9];
 #pragma unroll
         for (int v = 0; v < 9; ++v) U[v] = a.Uc[v * n + c];
-        rmhd_c2p<sizeof(F) == 4>(cp, U, q, st);
+        C2PStats sc = {0, 0, 0};
+        rmhd_c2p<sizeof(F) == 4>(cp, U, q, sc);
 #pragma unroll
         for (int v = 0; v < 9; ++v) a.Pout[v * n + c] = q[v];
+        if (a.flags && a.stage == a.nst - 1) a.flags[c] = (unsigned char)sc.flags;
+        st.hydro_iters += sc.hydro_iters;
+        st.newton_iters += sc.newton_iters;
+        st.flags |= sc.flags;
     }
     if (a.stats) wave_add_stats(a.stats, st);
 }

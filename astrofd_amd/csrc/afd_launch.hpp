@@ -18,6 +18,8 @@ struct FaceLaunch {
     int lim, recon, riemann, dir, diss;
     bool mixed;
     bool shared_edges;  // FP32 RMHD WENO5: the shared-edge kernels (afd_params.face_path == 0)
+    float *D;           // FP32 RMHD WENO5 with shared edges: the flux-divergence accumulator (else null)
+    float idn;          // 1 / spacing along dir
 };
 
 template <typename T> inline FaceArgs<T> face_args(const FaceLaunch &l) {
@@ -32,6 +34,8 @@ template <typename T> inline FaceArgs<T> face_args(const FaceLaunch &l) {
     a.v_max = l.v_max;
     a.lim = l.lim;
     a.diss = l.diss;
+    a.D = l.D;
+    a.idn = l.idn;
     if constexpr (sizeof(T) == 8) a.ph = l.phd;
     else a.ph = l.phf;
     const int *r0 = l.g.r0, *r1 = l.g.r1;
@@ -57,6 +61,12 @@ struct UpdLaunch {
     int nb;
 };
 hipError_t launch_update(const UpdLaunch &l, bool rhs_only, hipStream_t s);
+
+// Mixed RMHD WENO5 on the flux-divergence path: the y faces fused with the stage update
+// (k_face_march_update), the ghost shell's update and the learned start's fallback queue.
+hipError_t launch_stage_fused(const FaceLaunch &l, const UpdLaunch &u, hipStream_t s);
+long shell_cells(const Grid &g);
+long fused_blocks(const Grid &g);
 
 
 }  // namespace afd

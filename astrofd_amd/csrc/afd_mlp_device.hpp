@@ -42,6 +42,9 @@ struct MlpDevW {
 };
 
 AFD_D MlpDevW mlp_view(const void *w) {
+    // an opaque copy of the pointer: a kernel that evaluates the network inside a loop would
+    // otherwise hoist the 26 loop-invariant weight operands (104 VGPRs) out of it and spill
+    asm volatile("" : "+s"(w));
     MlpDevW v;
     v.blk = (const bf16x8 *)w;
     v.bias = (const float *)((const char *)w + (size_t)MLP_NBLK * 64 * 16);
@@ -158,7 +161,9 @@ AFD_D void mlp_infer_wave(const MlpDevW &w, const float x[5], float y[2]) {
     // one tile after the other: interleaved by the scheduler, the two tiles' activations took the
     // stage update to 163 registers (3 waves per SIMD); one at a time it needs 121 (4 waves)
     __builtin_amdgcn_sched_barrier(0);
-    mlp_tile_forward(w, lane, input_operand(xs, upper), t1);
+    MlpDevW w1 = w;  // the second tile loads the weights again rather than keep the first tile's live
+    asm volatile("" : "+s"(w1.blk));
+    mlp_tile_forward(w1, lane, input_operand(xs, upper), t1);
     float a0 = __shfl_xor(t1.o[0], 32), a1 = __shfl_xor(t1.o[1], 32);
     y[0] = upper ? a0 : t0.o[0];
     y[1] = upper ? a1 : t0.o[1];

@@ -34,8 +34,38 @@ template <int RIEM> static void go_weno_xy_f32(const FaceLaunch &l, hipStream_t 
     else hipLaunchKernelGGL((k_face_rmhd_weno_rows<RIEM, 1>), grid, block, 0, s, a);
 }
 
+// FP32 WENO5 flux divergence, z: waves of 62 cells overlapping by two positions.
+template <int RIEM> static void go_weno_z_div(const FaceLaunch &l, hipStream_t s) {
+    FaceArgs<float> a = face_args<float>(l);
+    const Grid &g = l.g;
+    const unsigned nz = g.r1[2] - g.r0[2], ny = g.r1[1] - g.r0[1], nx = g.r1[0] - g.r0[0];
+    const unsigned ncz = nz + 2, npos = ncz * ny * nx, waves = (npos - 1 + 61) / 62;
+    a.dk = make_udiv(ncz);
+    a.dn = make_udiv(ny);
+    hipLaunchKernelGGL((k_face_rmhd_weno_lanes_div<RIEM>), grid1d(64L * waves), dim3(BLOCK), 0, s, a);
+}
+
+// FP32 WENO5 flux divergence, x and y: four-wave pencils sweeping runs of MARCH_CELLS cells.
+template <int RIEM> static void go_weno_xy_march(const FaceLaunch &l, hipStream_t s) {
+    FaceArgs<float> a = face_args<float>(l);
+    const Grid &g = l.g;
+    const unsigned nz = g.r1[2] - g.r0[2], nkc = (nz + 63) / 64;
+    const unsigned nruns = (g.r1[l.dir] - g.r0[l.dir] + MARCH_CELLS - 1) / MARCH_CELLS;
+    const unsigned no = l.dir == 0 ? g.r1[1] - g.r0[1] : g.r1[0] - g.r0[0];
+    a.dk = make_udiv(nkc);
+    a.dn = make_udiv(nruns);
+    const dim3 grid(nkc * nruns * no), block(256);
+    if (l.dir == 0) hipLaunchKernelGGL((k_face_rmhd_weno_march<RIEM, 0>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_face_rmhd_weno_march<RIEM, 1>), grid, block, 0, s, a);
+}
+
 template <int RECON, int RIEM, typename T> static void go(const FaceLaunch &l, hipStream_t s) {
     if constexpr (RECON == 2 && sizeof(T) == 4) {
+        if (l.D) {  // the context decided: shared edges, below 2^29 cells
+            if (l.dir == 2) go_weno_z_div<RIEM>(l, s);
+            else go_weno_xy_march<RIEM>(l, s);
+            return;
+        }
         if (l.shared_edges && l.g.ncell < (1L << 29)) {
             if (l.dir == 2) go_weno_z_f32<RIEM>(l, s);
             else go_weno_xy_f32<RIEM>(l, s);

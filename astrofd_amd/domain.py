@@ -343,6 +343,15 @@ class Domain:
         L.check(self.lib.afd_get_stats(self.h, ctypes.byref(st)), "afd_get_stats")
         return st.as_dict()
 
+    def c2p_flags(self):
+        """Per-cell con2prim branch bits of the last stage of the latest step run with stats on
+        (bit 0 hydro energy fallback, 1 bisection, 2 invalid recovery, 3 learned/previous start)."""
+        n = int(np.prod(self.shape[1:]))
+        out = np.zeros(n, dtype=np.uint8)
+        L.check(self.lib.afd_get_c2p_flags(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), n),
+                "afd_get_c2p_flags")
+        return out.reshape(self.shape[1:])
+
     def reset_stats(self):
         L.check(self.lib.afd_reset_stats(self.h), "afd_reset_stats")
 

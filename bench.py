@@ -22,7 +22,17 @@ Prints ONE JSON line.  Also: the dominant kernel's roofline from HIP events on
 the library's stream over the timed steps, the solver counters of one extra
 step, the fused post-step source kernel (cooling, two-temperature, chemistry)
 on the same grid, and the CPU oracle's rate on a bounded sample on this host's
-cores.
+cores.  Beside the headline:
+  fp64_ulp_spread  the plain FP64 step's own response to a 1-ulp perturbation of its input
+                   (the reference's hll_d_flux_dir is ill-conditioned over a static ambient)
+  hlle_50          50 consecutive steps of the same jet with RIEMANN = "hlle" (the reference's
+                   own WENO RMHD choice, stable on this jet): mixed + learned against plain FP64
+  network          the learned guess's MLP alone over 2^24 samples: achieved BF16 MFMA rate
+  srhd_jet_256     the SRHD 256^3 jet (MUSCL-van Leer + HLLC + SSPRK3): timed plain and mixed
+                   steps, errors after steps 1, 10 and 100, and the cells whose con2prim took the
+                   other branch (converged vs energy fallback) in the mixed run
+  rmhd_jet_512     the 512^3 RMHD jet with resistivity and Te/Ti: timed plain and mixed + learned
+                   steps and the errors after one step
 """
 import argparse
 import json
@@ -159,39 +169,83 @@ def run_timed(s, pr0, steps, warmup, mlp=None, repeat=True, face_path=0):
         d.close()
 
 
-def accuracy(sp, sm, pr0, checkpoints, mlp=None):
+def accuracy(sp, sm, pr0, checkpoints, mlp=None, flips=False):
     """Plain and mixed runs in lockstep from the same state; errors of mixed against plain after
-    each checkpoint step (cells where both are finite), and the non-finite counts."""
+    each checkpoint step (cells where both are finite), and the non-finite counts.  flips: also
+    count the cells whose last-stage con2prim took the other branch of the hydro fixed point
+    (converged vs energy fallback) in the two runs, at each checkpoint step."""
     dp, rp = open_domain(sp, pr0)
     dm, rm = open_domain(sm, pr0, mlp)
     out = []
     try:
         step_p, step_m = make_stepper(dp, sp, rp), make_stepper(dm, sm, rm)
         for n in range(1, max(checkpoints) + 1):
+            cp = n in checkpoints
+            if cp and flips:
+                dp.set_instrumentation(stats=True, timing=False)
+                dm.set_instrumentation(stats=True, timing=False)
             step_p()
             step_m()
-            if n in checkpoints:
+            if cp:
                 a, b = dm.get_prim(), dp.get_prim()
                 e = errors(a, b, sp["NG"])
                 e["step"] = n
                 e["nonfinite"] = {"plain": int(np.count_nonzero(~np.isfinite(b))),
                                   "mixed": int(np.count_nonzero(~np.isfinite(a)))}
+                del a, b
+                if flips:
+                    ng = sp["NG"]
+                    sl = (slice(ng, -ng),) * 3
+                    fp, fm = dp.c2p_flags()[sl] & 1, dm.c2p_flags()[sl] & 1
+                    e["c2p_branch_flips"] = int(np.count_nonzero(fp != fm))
+                    e["c2p_fallback_cells"] = {"plain": int(fp.sum()), "mixed": int(fm.sum())}
+                    dp.set_instrumentation(stats=False, timing=False)
+                    dm.set_instrumentation(stats=False, timing=False)
                 out.append(e)
-                log(f"[bench]   accuracy after step {n}: L1(rho) {e['rho']['L1_rel']:.3e}, nonfinite {e['nonfinite']}")
+                log(f"[bench]   accuracy after step {n}: L1(rho) {e['rho']['L1_rel']:.3e}, nonfinite {e['nonfinite']}"
+                    + (f", con2prim branch flips {e['c2p_branch_flips']}" if flips else ""))
     finally:
         dp.close()
         dm.close()
     return out
 
 
+def ulp_spread(sp, pr0):
+    """The plain FP64 step's response to its own rounding: one step from the seeded state and one
+    from the same state perturbed by 1 ulp (seeded signs), errors of the second against the first."""
+    pert = pr0 * (1.0 + 2.220446049250313e-16 * np.random.default_rng(99).choice([-1.0, 1.0], pr0.shape))
+    res = []
+    for init in (pr0, pert):
+        d, rng = open_domain(sp, init)
+        try:
+            make_stepper(d, sp, rng)()
+            res.append(d.get_prim())
+        finally:
+            d.close()
+    e = errors(res[1], res[0], sp["NG"])
+    e["step"] = 1
+    return e
+
+
 def measured_traffic(workload):
     """HBM bytes per launch by kernel, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
-    (profiles/<workload>_traffic.txt, scaled by a plain-copy calibration), or {}."""
+    (profiles/<workload>_traffic.txt, scaled by a plain-copy calibration), and its source, or
+    ({}, None)."""
     path = os.path.join(ROOT, "profiles", f"{workload}_traffic.txt")
     if not os.path.exists(path):
-        return {}
+        return {}, None
     with open(path) as f:
-        return json.load(f).get("traffic_bytes_per_launch", {})
+        doc = json.load(f)
+    src = f"profiles/{workload}_traffic.txt: " + doc.get("source", "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE")
+    return doc.get("traffic_bytes_per_launch", {}), src
+
+
+def flux_divergence_path(s, mixed):
+    """The mixed RMHD WENO5 path on grids below 2^29 cells: the face kernels accumulate the flux
+    divergence (astrofd_amd/csrc/afd_kernels.hpp, k_face_rmhd_weno_lanes_div / _march)."""
+    from astrofd_amd.domain import recon_id
+    cells = (s["NX"] + 2 * s["NG"]) * (s["NY"] + 2 * s["NG"]) * (s["NZ"] + 2 * s["NG"])
+    return s["PHYSICS"] == "rmhd" and mixed and recon_id(s["RECON"]) == 2 and cells < (1 << 29)
 
 
 def kernel_name(s, which, mixed, learned):
@@ -203,51 +257,97 @@ def kernel_name(s, which, mixed, learned):
     from astrofd_amd.domain import recon_id, riemann_id
     d = {"face_x": 0, "face_y": 1, "face_z": 2}[which]
     rid = riemann_id(s['PHYSICS'], s['RIEMANN'])
-    cells = (s["NX"] + 2 * s["NG"]) * (s["NY"] + 2 * s["NG"]) * (s["NZ"] + 2 * s["NG"])
-    if nb == 9 and mixed and recon_id(s['RECON']) == 2 and cells < (1 << 29):
-        # FP32 WENO5 faces sharing each cell's edge values (LDS rows for x/y, lane shuffle for z)
-        return f"k_face_rmhd_weno_lanes<{rid}>" if d == 2 else f"k_face_rmhd_weno_rows<{rid}, {d}>"
+    if flux_divergence_path(s, mixed):
+        # FP32 WENO5 faces accumulating the flux divergence (pencils for x/y, lanes for z)
+        return f"k_face_rmhd_weno_lanes_div<{rid}>" if d == 2 else f"k_face_rmhd_weno_march<{rid}, {d}>"
     kind = "k_face_rmhd" if nb == 9 else "k_face_srhd"
     return f"{kind}<{recon_id(s['RECON'])}, {rid}, {d}, {T}>"
+
+
+def nvars(s):
+    nb = 9 if s["PHYSICS"] == "rmhd" else 5
+    return nb + int(s.get("N_PASSIVE", 0))
+
+
+def algorithmic_update_bytes(s):
+    """Compulsory HBM bytes per cell of one stage update as the metric counts them: an ideally
+    fused SSPRK stage reads P_s-1, U0 and U_s-1 and writes U_s and P_s, 13 state arrays a step for
+    SSPRK3 and 8 for SSPRK2, each nv x 8 B, spread over the stages."""
+    nst = 3 if s["RK_ORDER"] == 3 else 2
+    return (13 if nst == 3 else 8) * nvars(s) * 8 / nst
+
+
+def impl_update_bytes(s, mixed):
+    """HBM bytes per cell of one stage update as this implementation moves them, averaged over the
+    stages: read P0 (U0 = p2c(P0) every stage), read U_s-1 (stages > 0), write U_s (not the last
+    stage), write P_s, read psi and the passives of P_s-1, and the face terms: one FP32 flux
+    divergence on the flux-divergence path, else one read of each face-flux array."""
+    nb = 9 if s["PHYSICS"] == "rmhd" else 5
+    nv = nvars(s)
+    nst = 3 if s["RK_ORDER"] == 3 else 2
+    fb = 4 if mixed else 8
+    carried = nb == 9 and s["RECON"] != "muscl"
+    face = nb * 4 if flux_divergence_path(s, mixed) else (4 if carried else 3) * nb * fb
+    per = []
+    for st in range(nst):
+        b = nv * 8 + nv * 8 + face + (8 if nb == 9 else 0) + (nv - nb) * 8
+        b += nb * 8 if st > 0 else 0
+        b += nb * 8 if st < nst - 1 else 0
+        per.append(b)
+    return sum(per) / nst
+
+
+def face_bytes(s, mixed, dkey):
+    """Compulsory bytes per face (flux path) or per cell (flux-divergence path) of a face kernel:
+    read the nb primitives of one cell (the stencil's other cells are re-reads) and write its output:
+    the face flux (twice for the RMHD PPM/WENO x faces, which also read the two carried speeds the
+    z kernel wrote), or read and write the FP32 divergence (the z kernel writes it and the speeds)."""
+    nb = 9 if s["PHYSICS"] == "rmhd" else 5
+    fb = 4 if mixed else 8
+    carried = nb == 9 and s["RECON"] != "muscl"
+    if flux_divergence_path(s, mixed):
+        return [nb * 8 + 2 * 4 + 2 * nb * 4, nb * 8 + 2 * nb * 4, nb * 8 + nb * 4 + 2 * 4][dkey]
+    b = nb * 8 + nb * fb
+    if carried and dkey == 0:
+        b += nb * fb + 2 * fb
+    if carried and dkey == 2:
+        b += 2 * fb
+    return b
 
 
 def kernel_roofline(s, kern, mixed, learned=False, workload=None):
     """Dominant kernel and its achieved algorithmic HBM bandwidth (bytes per launch / avg duration).
 
-    Algorithmic bytes per launch (compulsory traffic of one launch over its units):
-      face kernel, per face: read the nb primitives of one cell (8 B each; the stencil's other cells
-        are re-reads) and write the face flux (nb x flux size; twice for the RMHD PPM/WENO x faces,
-        which also read the two carried speeds the z kernel wrote there).
-      stage update, one thread per cell (ghosts included): per stage 4 x nb x 8 B of conserved and
-        primitive state (stage 0: read P0, write U0, U1, P1; later stages: read U0, U_s-1, write U_s,
-        P_s), the psi of the stage input (8 B) and one read of each face-flux array (3 arrays, 4 with
-        the carried x fluxes).
+    Stage update (one thread per cell, ghosts included): the metric's compulsory state traffic per
+    stage (algorithmic_update_bytes) times the cells of a launch; impl_bytes_per_launch is what
+    this implementation moves (impl_update_bytes).  Face kernels: face_bytes per face (or per cell
+    on the flux-divergence path).  traffic: HBM bytes per launch measured by rocprofv3 in the
+    committed profile named by traffic_source.
     """
-    nb = 9 if s["PHYSICS"] == "rmhd" else 5
-    fb = 4 if mixed else 8
     NX, NY, NZ = s["NX"], s["NY"], s["NZ"]
     full = (NX + 2 * s["NG"]) * (NY + 2 * s["NG"]) * (NZ + 2 * s["NG"])
-    carried = nb == 9 and s["RECON"] != "muscl"
-    per_face = [nb * 8 + nb * fb, nb * 8 + nb * fb, nb * 8 + nb * fb]
-    if carried:
-        per_face[0] += nb * fb + 2 * fb
-        per_face[2] += 2 * fb
-    faces = [(NX + 1) * NY * NZ, NX * (NY + 1) * NZ, NX * NY * (NZ + 1)]
-    upd_per_cell = 4 * nb * 8 + 8 + (4 if carried else 3) * nb * fb
+    units = [(NX + 1) * NY * NZ, NX * (NY + 1) * NZ, NX * NY * (NZ + 1)]
+    if flux_divergence_path(s, mixed):
+        units = [NX * NY * NZ] * 3
     cands = []
     for dkey, name in ((0, "face_x"), (1, "face_y"), (2, "face_z")):
         n = kern["launches_face"][dkey]
         if n:
-            cands.append((kern["ms_face"][dkey] / n, name, per_face[dkey] * faces[dkey]))
+            nbytes = face_bytes(s, mixed, dkey) * units[dkey]
+            cands.append((kern["ms_face"][dkey] / n, name, nbytes, nbytes))
     if kern["launches_update"]:
-        cands.append((kern["ms_update"] / kern["launches_update"], "stage_update_c2p", upd_per_cell * full))
-    ms, name, nbytes = max(cands)
+        cands.append((kern["ms_update"] / kern["launches_update"], "stage_update_c2p",
+                      algorithmic_update_bytes(s) * full, impl_update_bytes(s, mixed) * full))
+    ms, name, nbytes, impl = max(cands)
     achieved = nbytes / (ms * 1e-3) / 1e9
     kname = kernel_name(s, name, mixed, learned)
-    traffic = measured_traffic(workload).get(kname) if workload else None
+    table, src = measured_traffic(workload) if workload else ({}, None)
+    traffic = table.get(kname)
     return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": round(traffic) if traffic else None,
-            "avg_ms": round(ms, 4), "bytes_per_launch": int(nbytes), "rocprof_name": kname}
+            "traffic_source": src if traffic else None, "avg_ms": round(ms, 4), "bytes_per_launch": int(nbytes),
+            "impl_bytes_per_launch": int(impl), "impl_frac": round(impl / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+            "rocprof_name": kname}
 
 
 def cpu_baseline(s, n=128, steps=2, n1=48):
@@ -339,6 +439,97 @@ def sources_roofline(s, pr0, launches=20):
             "bytes_per_launch": int(nbytes), "rocprof_name": "k_sources(SourceArgs)"}
 
 
+MLP_FLOP_PER_SAMPLE = 2 * (5 * 64 + 64 * 64 + 64 * 2)  # 9,088: three dense layers, multiply + add
+BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
+
+
+def network_rate(mlp, n=1 << 24, launches=3):
+    """The learned guess's MLP (5 -> 64 -> 64 -> 2, SiLU, BF16 weights, FP32 accumulation) alone over
+    n samples: kernel time from HIP events on the library's stream, achieved FLOP rate against the
+    dense BF16 MFMA peak.  The inputs are uniform in [-1, 1] (the rate does not depend on them)."""
+    from astrofd_amd.domain import Domain
+    s = dict(load_workload(DEFAULT_WORKLOAD), NX=8, NY=8, NZ=8, PRECISION="mixed", C2P_GUESS="learned")
+    X = np.random.default_rng(5).uniform(-1.0, 1.0, (n, 5)).astype(np.float32)
+    d = Domain.from_settings(s)
+    try:
+        d.set_mlp_weights(mlp)
+        d.mlp_forward(X[:1024])
+        d.set_instrumentation(stats=False, timing=True)
+        d.reset_stats()
+        for _ in range(launches):
+            d.mlp_forward(X)
+        ms = d.stats()["ms_other"] / launches
+        d.set_instrumentation(stats=False, timing=False)
+    finally:
+        d.close()
+    tflops = MLP_FLOP_PER_SAMPLE * n / (ms * 1e-3) / 1e12
+    return {"kernel": "k_mlp_forward", "samples": n, "avg_ms": round(ms, 4), "flop_per_sample": MLP_FLOP_PER_SAMPLE,
+            "achieved": round(tflops, 2), "peak": BF16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tflops / BF16_MFMA_PEAK_TFLOPS, 5)}
+
+
+def timed_entry(res, zu, steps):
+    return {"value": round(zu / res["elapsed"], 1), "ms_per_step": round(res["elapsed"] * 1e3 / steps, 3),
+            "kernel_ms_per_step": kernel_ms(res["kernels"], steps), "c2p": c2p_report(res["stats"])}
+
+
+def hlle_leg(s, pr0, mlp, nsteps=50, checkpoints=(1, 10, 25, 50)):
+    """The same jet with RIEMANN = "hlle" (stable on it), 50 consecutive steps: plain FP64 and mixed
+    (+ learned guess) timed as consecutive runs, and their errors after the checkpoint steps."""
+    sh = dict(s, RIEMANN="hlle")
+    sp = dict(sh, PRECISION="fp64", C2P_GUESS="plain")
+    sm = dict(sh, PRECISION="mixed", C2P_GUESS="learned" if mlp is not None else "plain")
+    zu = s["NX"] * s["NY"] * s["NZ"] * nsteps
+    log(f"[bench] HLLE companion: {nsteps} consecutive steps, plain and mixed")
+    p = run_timed(sp, pr0, nsteps, 0, repeat=False)
+    m = run_timed(sm, pr0, nsteps, 0, mlp=mlp, repeat=False)
+    log(f"[bench]   plain {p['elapsed'] * 1e3 / nsteps:.2f} ms/step, mixed {m['elapsed'] * 1e3 / nsteps:.2f} ms/step")
+    ev = accuracy(sp, sm, pr0, set(checkpoints), mlp)
+    return {"workload": f"{DEFAULT_WORKLOAD} with RIEMANN=hlle, {nsteps} consecutive steps",
+            "fp64_plain": timed_entry(p, zu, nsteps), "mixed": timed_entry(m, zu, nsteps),
+            "speedup_vs_fp64_plain": round(p["elapsed"] / m["elapsed"], 3), "errors_vs_fp64_plain": ev}
+
+
+def config_bench(name, steps, warmup, checkpoints, mlp, flips=False):
+    """Another benchmark config on one GPU: timed plain FP64 and mixed steps (every step from the
+    seeded state), each run's dominant-kernel roofline, and the mixed run's errors against the plain
+    one after the checkpoint steps."""
+    from astrofd_amd import ic
+    s = load_workload(name)
+    t0 = time.perf_counter()
+    pr0 = ic.initial_state(s)
+    log(f"[bench] {name}: initial state {pr0.shape} in {time.perf_counter() - t0:.1f}s")
+    rm = s["PHYSICS"] == "rmhd"
+    use_mlp = mlp if rm else None
+    sp = dict(s, PRECISION="fp64", C2P_GUESS="plain")
+    sm = dict(s, PRECISION="mixed", C2P_GUESS="learned" if use_mlp is not None else "plain")
+    zu = s["NX"] * s["NY"] * s["NZ"] * steps
+    p = run_timed(sp, pr0, steps, warmup)
+    log(f"[bench]   {name} plain {p['elapsed'] * 1e3 / steps:.2f} ms/step")
+    m = run_timed(sm, pr0, steps, warmup, mlp=use_mlp)
+    log(f"[bench]   {name} mixed {m['elapsed'] * 1e3 / steps:.2f} ms/step")
+    out = {"workload": f"{name}: {workload_desc(s)}", "steps": steps, "warmup": warmup,
+           "fp64_plain": dict(timed_entry(p, zu, steps), roofline=kernel_roofline(s, p["kernels"], False, False, name)),
+           "mixed": dict(timed_entry(m, zu, steps), c2p_guess=sm["C2P_GUESS"],
+                         roofline=kernel_roofline(s, m["kernels"], True, use_mlp is not None, name)),
+           "speedup_vs_fp64_plain": round(p["elapsed"] / m["elapsed"], 3)}
+    out["errors_vs_fp64_plain"] = accuracy(sp, sm, pr0, set(checkpoints), use_mlp, flips=flips)
+    return out
+
+
+def workload_desc(s):
+    rm = s["PHYSICS"] == "rmhd"
+    recon = {"weno": "WENO5", "ppm": "PPM"}.get(str(s["RECON"]).lower(), f"MUSCL({s['LIMITER']})")
+    riem = "HLLD(hll_d_flux_dir)" if rm and s["RIEMANN"] == "hlld_full" else str(s["RIEMANN"]).upper()
+    extra = ""
+    if s.get("RESISTIVE_ENABLED"):
+        extra += f", resistivity {s['RESISTIVITY']:g}"
+    if s.get("TWO_TEMPERATURE"):
+        extra += f", Te/Ti (TEI_TAU {s['TEI_TAU']:g})"
+    return (f"{'RMHD' if rm else 'SRHD'} {s['NX']}^3 {recon} + {riem} + SSPRK{s['RK_ORDER']}, "
+            f"Gamma={s['GAMMA']:.4g}, CFL {s['CFL']}{extra}")
+
+
 def kernel_ms(k, timing_steps):
     return {"face_x": round(k["ms_face"][0] / timing_steps, 3), "face_y": round(k["ms_face"][1] / timing_steps, 3),
             "face_z": round(k["ms_face"][2] / timing_steps, 3),
@@ -385,6 +576,9 @@ def main():
     ap.add_argument("--evolve-steps", type=int, default=10,
                     help="consecutive steps timed beside the headline (0 = skip)")
     ap.add_argument("--copy-probe", type=float, default=0.0, help="GiB of a plain device copy (profiling)")
+    ap.add_argument("--no-hlle", action="store_true", help="skip the 50-step HLLE companion")
+    ap.add_argument("--no-configs", action="store_true", help="skip the SRHD 256^3 and RMHD 512^3 entries")
+    ap.add_argument("--no-network", action="store_true", help="skip the MLP rate measurement")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}): the step is built and measured for one "
@@ -442,10 +636,7 @@ def main():
     value = zu / mixed["elapsed"]
     nb = 9 if s["PHYSICS"] == "rmhd" else 5
     rm = s["PHYSICS"] == "rmhd"
-    recon = {"weno": "WENO5", "ppm": "PPM"}.get(str(s["RECON"]).lower(), f"MUSCL({s['LIMITER']})")
-    riem = "HLLD(hll_d_flux_dir)" if rm and s["RIEMANN"] == "hlld_full" else str(s["RIEMANN"]).upper()
-    desc = (f"{'RMHD' if rm else 'SRHD'} {s['NX']}^3 {recon} + {riem} + SSPRK{s['RK_ORDER']}, "
-            f"Gamma={s['GAMMA']:.4g}, CFL {s['CFL']}")
+    desc = workload_desc(s)
     res = {
         "metric": ("zone-updates/s RMHD 256³ WENO5+HLLD: mixed+learned vs FP64 plain at L1≤1e-6"
                    if a.workload == DEFAULT_WORKLOAD else f"zone-updates/s {desc}: mixed vs FP64 plain"),
@@ -461,7 +652,7 @@ def main():
     }
     res["roofline"] = kernel_roofline(s, mixed["kernels"], True, mlp is not None, a.workload)
     res["kernel_ms_per_step"] = kernel_ms(mixed["kernels"], a.steps)
-    step_bytes = (13 if s["RK_ORDER"] == 3 else 8) * nb * 8
+    step_bytes = (13 if s["RK_ORDER"] == 3 else 8) * nvars(s) * 8
     res["step_roofline"] = {"bytes_per_zone_update": step_bytes, "achieved": round(step_bytes * value / 1e9, 2),
                             "unit": "GB/s", "frac": round(step_bytes * value / 1e9 / HBM_PEAK_GBS, 5)}
     res["c2p"] = c2p_report(mixed["stats"])
@@ -494,6 +685,8 @@ def main():
             # show how the reference scheme's own divergence (non-finite counts) takes over
             res["errors_vs_fp64_plain"] = ev[0]
             res["errors_evolution"] = ev
+            log("[bench] FP64 response to 1-ulp input noise (one step)")
+            res["fp64_ulp_spread"] = ulp_spread(sp, pr0)
         if a.evolve_steps > 0 and not a.evolve:
             zu_e = s["NX"] * s["NY"] * s["NZ"] * a.evolve_steps
             log(f"[bench] {a.evolve_steps} consecutive steps (plain, mixed)")
@@ -502,9 +695,18 @@ def main():
             res["consecutive_steps"] = {"steps": a.evolve_steps, "fp64_plain": round(zu_e / ep["elapsed"], 1),
                                         "mixed": round(zu_e / em["elapsed"], 1),
                                         "mixed_c2p": c2p_report(em["stats"])}
+    if rm and not a.no_hlle and a.workload == DEFAULT_WORKLOAD:
+        res["hlle_50"] = hlle_leg(s, pr0, mlp)
+    if mlp is not None and not a.no_network:
+        log("[bench] MLP forward rate")
+        res["network"] = network_rate(mlp)
     if not a.no_sources:
         log("[bench] post-step sources kernel")
         res["post_step_sources"] = sources_roofline(s, pr0)
+    if not a.no_configs and a.workload == DEFAULT_WORKLOAD:
+        del pr0
+        res["srhd_jet_256"] = config_bench("srhd_jet_256", 10, 2, (1, 10, 100), None, flips=True)
+        res["rmhd_jet_512"] = config_bench("rmhd_jet_512", 3, 1, (1,), mlp)
     if not a.no_cpu_baseline:
         log("[bench] CPU oracle sample")
         res["cpu_baseline"] = cpu_baseline(s)

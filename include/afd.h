@@ -239,6 +239,11 @@ int afd_centerline(afd_ctx *ctx, double *out, int64_t n);
 int afd_set_instrumentation(afd_ctx *ctx, int32_t stats, int32_t timing);
 int afd_get_stats(afd_ctx *ctx, afd_stats *out);
 int afd_reset_stats(afd_ctx *ctx);
+/* Per-cell con2prim branch of the last stage of the latest step run with the counters on
+ * (no counterpart in the reference): out = one byte per cell (ghosts included), bit 0 hydro
+ * fixed point took the energy fallback, bit 1 RMHD bisection, bit 2 RMHD invalid recovery (hydro
+ * guess kept), bit 3 learned or previous-state start accepted. */
+int afd_get_c2p_flags(afd_ctx *ctx, uint8_t *out, int64_t n);
 int afd_synchronize(afd_ctx *ctx);
 
 /* Learned con2prim guess (no counterpart in the reference): an MLP 5 -> 64 -> 64 -> 2 (SiLU)

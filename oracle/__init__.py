@@ -32,6 +32,7 @@ class Params(ctypes.Structure):
         ("resistive", ctypes.c_int), ("resistivity", ctypes.c_double),
         ("gr_metric", ctypes.c_int), ("gr_orthonormal", ctypes.c_int), ("gr_x0", ctypes.c_int),
         ("gr_mass", ctypes.c_double), ("gr_gamma", ctypes.c_double),
+        ("resist_once", ctypes.c_int),
     ]
 
 
@@ -147,7 +148,8 @@ def riemann_id(physics, name):
 
 def make_params(physics="hydro", shape=(1, 1, 1), nv=None, ng=2, gamma=None, p_max=1.0, v_max=0.999,
                 glm_ch=1.0, glm_cp=0.1, limiter="mc", recon="muscl", riemann="hlle", rk_order=2,
-                n_passive=0, passive_offset=None, resistive=False, resistivity=0.0, gr_metric="minkowski",
+                n_passive=0, passive_offset=None, resistive=False, resistivity=0.0, resist_once=False,
+                gr_metric="minkowski",
                 gr_mass=1.0, orthonormal_flux=True, gr_gamma=None, gr_x0=0):
     """Parameters as the reference's configure() leaves them (core/srhd_core.py:18-41,
     core/rmhd_core.py:21-50).  SRHD always computes with Gamma = 5/3 (Q5)."""
@@ -165,7 +167,7 @@ def make_params(physics="hydro", shape=(1, 1, 1), nv=None, ng=2, gamma=None, p_m
                   glm_ch=glm_ch, glm_cp=glm_cp, limiter_id=LIMITERS.get(limiter, 0), recon_id=RECONS.get(recon, 0),
                   riemann_id=riemann_id(physics, riemann), rk_order=rk_order, n_passive=n_passive,
                   passive_offset=passive_offset if passive_offset is not None else nb,
-                  resistive=int(bool(resistive)), resistivity=resistivity)
+                  resistive=int(bool(resistive)), resistivity=resistivity, resist_once=int(bool(resist_once)))
 
 
 def with_shape(P, pr):

@@ -1103,7 +1103,7 @@ static void rmhd_rhs_ppm_weno(const orc_params *P, const double *pr, double *rhs
                               double dz) {
     const int i0 = 3, i1 = NXF - 3, j0 = 3, j1 = NYF - 3, k0 = 3, k1 = NZF - 3;
     const double dd[3] = {dx, dy, dz};
-    const int n_resist = (P->recon_id == 1) ? 2 : 0;
+    const int n_resist = P->resist_once ? 1 : ((P->recon_id == 1) ? 2 : 0);
 #pragma omp parallel for schedule(static)
     for (int i = i0; i < i1; ++i) {
         double sL = -1.0, sR = 1.0;

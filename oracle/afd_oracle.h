@@ -42,6 +42,8 @@ typedef struct orc_params {
        2 kerr-schild, 3 any other GR_METRIC (sources from the Kerr-Schild lapse, no rescale) */
     int gr_metric, gr_orthonormal, gr_x0;
     double gr_mass, gr_gamma; /* GR_MASS; the Gamma of the source's enthalpy (GRHD: config GAMMA) */
+    int resist_once;          /* additive RESISTIVE_MODE "once": the Laplacian once per RHS in every scheme
+                                 (the reference: once in MUSCL, twice in PPM, never in WENO) */
 } orc_params;
 
 /* pointwise (batched over n states, row-major n x nvar) */

@@ -22,7 +22,8 @@ def params_from_settings(s, shape):
                          glm_ch=s.get("GLM_CH", 1.0), glm_cp=s.get("GLM_CP", 0.1), limiter=s["LIMITER"],
                          recon=s["RECON"], riemann=s["RIEMANN"], rk_order=s["RK_ORDER"],
                          n_passive=s.get("N_PASSIVE", 0), passive_offset=s.get("PASSIVE_OFFSET"),
-                         resistive=s.get("RESISTIVE_ENABLED", False), resistivity=s.get("RESISTIVITY", 0.0))
+                         resistive=s.get("RESISTIVE_ENABLED", False), resistivity=s.get("RESISTIVITY", 0.0),
+                         resist_once=str(s.get("RESISTIVE_MODE", "reference")).lower() == "once")
 
 
 def apply_bcs(P, s, pr, rng, b_amb=(0.0, 0.0, 0.0), bc_y="periodic", bc_z="periodic"):

@@ -19,7 +19,7 @@ def test_more_than_one_gpu_is_refused():
 
 
 @pytest.mark.parametrize("mixed,learned,expect", [
-    (True, True, ["k_face_rmhd_weno_rows<2, 0>", "k_face_rmhd_weno_rows<2, 1>", "k_face_rmhd_weno_lanes<2>",
+    (True, True, ["k_face_rmhd_weno_march<2, 0>", "k_face_rmhd_weno_march<2, 1>", "k_face_rmhd_weno_lanes_div<2>",
                   "k_update<9, float, true>"]),
     (False, False, ["k_face_rmhd<2, 2, 0, double>", "k_face_rmhd<2, 2, 1, double>", "k_face_rmhd<2, 2, 2, double>",
                     "k_update<9, double, false>"]),
@@ -27,7 +27,7 @@ def test_more_than_one_gpu_is_refused():
 def test_kernel_names_of_the_rmhd_workload(mixed, learned, expect):
     s = bench.load_workload("rmhd_jet_256")
     got = [bench.kernel_name(s, w, mixed, learned) for w in ("face_x", "face_y", "face_z", "stage_update_c2p")]
-    # the FP32 WENO5 faces are the shared-edge kernels (rows for x/y, lanes for z)
+    # the FP32 WENO5 faces accumulate the flux divergence (pencils for x/y, lanes for z)
     assert got == expect
 
 
@@ -37,17 +37,51 @@ def test_kernel_names_of_the_srhd_workload():
     assert bench.kernel_name(s, "stage_update_c2p", True, False) == "k_update<5, float, false>"
 
 
+def test_algorithmic_update_bytes_are_the_metrics():
+    """13 state arrays of nv x 8 B per zone-update for SSPRK3 (8 for SSPRK2), spread over the stages:
+    936 / 3 = 312 B a cell per stage for RMHD, 1144 / 3 for the 512^3 jet with Te/Ti, 520 / 3 for
+    SRHD, 320 / 2 for the SSPRK2 SRHD jet."""
+    assert bench.algorithmic_update_bytes(bench.load_workload("rmhd_jet_256")) == 312
+    assert bench.algorithmic_update_bytes(bench.load_workload("rmhd_jet_512")) == pytest.approx(1144 / 3)
+    assert bench.algorithmic_update_bytes(bench.load_workload("srhd_jet_256")) == pytest.approx(520 / 3)
+    assert bench.algorithmic_update_bytes(bench.load_workload("srhd_jet_32")) == 160
+
+
+def test_impl_update_bytes():
+    """RMHD SSPRK3, mixed, flux-divergence path: stage 0 reads P0, writes U1 and P1; stage 1 also
+    reads U1; stage 2 reads U2 and writes no U; each reads psi and the 9 x 4 B divergence."""
+    s = bench.load_workload("rmhd_jet_256")
+    stage = [72 + 72 + 72 + 36 + 8, 72 + 72 + 72 + 72 + 36 + 8, 72 + 72 + 72 + 36 + 8]
+    assert bench.impl_update_bytes(s, True) == pytest.approx(sum(stage) / 3)
+    # FP64: four 9 x 8 B flux reads instead of the divergence
+    stage = [b - 36 + 4 * 72 for b in stage]
+    assert bench.impl_update_bytes(s, False) == pytest.approx(sum(stage) / 3)
+
+
 def test_roofline_picks_the_slowest_kernel_and_counts_its_bytes():
-    """RMHD WENO5 at 256^3, mixed: x faces move 9 x 8 B of primitives, two 9 x 4 B fluxes and two
-    4 B carried speeds per face over 257 x 256 x 256 faces; the update 4 x 9 x 8 B of state, 8 B
-    of psi and four 9 x 4 B flux reads per cell over 262^3 cells."""
+    """RMHD WENO5 at 256^3, mixed: the update's bytes per launch are the metric's 312 B a cell per
+    stage over 262^3 cells; the x pencils move 9 x 8 B of primitives, 2 x 4 B of carried speeds and
+    read and write 9 x 4 B of divergence per cell over 256^3 cells."""
     s = bench.load_workload("rmhd_jet_256")
     kern = {"launches_face": [3, 3, 3], "ms_face": [3.0, 2.4, 2.1], "launches_update": 3, "ms_update": 6.3}
     r = bench.kernel_roofline(s, kern, True, True)
     assert r["kernel"] == "stage_update_c2p" and r["avg_ms"] == pytest.approx(2.1)
-    assert r["bytes_per_launch"] == (4 * 9 * 8 + 8 + 4 * 9 * 4) * 262 ** 3
+    assert r["bytes_per_launch"] == 312 * 262 ** 3
+    assert r["impl_bytes_per_launch"] == int(bench.impl_update_bytes(s, True) * 262 ** 3)
     assert r["frac"] == pytest.approx(r["achieved"] / bench.HBM_PEAK_GBS, rel=1e-4)
+    assert r["achieved"] == pytest.approx(312 * 262 ** 3 / 2.1e-3 / 1e9, rel=1e-4)
     kern["ms_face"][0] = 9.0  # x faces slowest now
     r = bench.kernel_roofline(s, kern, True, True)
     assert r["kernel"] == "face_x"
-    assert r["bytes_per_launch"] == (9 * 8 + 2 * 9 * 4 + 2 * 4) * 257 * 256 * 256
+    assert r["bytes_per_launch"] == (9 * 8 + 2 * 4 + 2 * 9 * 4) * 256 ** 3
+    r = bench.kernel_roofline(s, kern, False, False)  # FP64: per face, two fluxes and the speeds
+    assert r["bytes_per_launch"] == (9 * 8 + 2 * 9 * 8 + 2 * 8) * 257 * 256 * 256
+
+
+def test_traffic_names_its_source():
+    s = bench.load_workload("rmhd_jet_256")
+    kern = {"launches_face": [3, 3, 3], "ms_face": [3.0, 2.4, 2.1], "launches_update": 3, "ms_update": 6.3}
+    r = bench.kernel_roofline(s, kern, True, True, "rmhd_jet_256")
+    assert (r["traffic"] is None) == (r["traffic_source"] is None)
+    if r["traffic"] is not None:
+        assert r["traffic_source"].startswith("profiles/rmhd_jet_256_traffic.txt")

@@ -9,8 +9,10 @@ ambient field added to the nozzle (B_AMB).  Both are pinned here by what they mu
                x ghost planes are filled the same way from their own values
   B_AMB        nozzle B = B0 s e_(poloidal: x, toroidal: y) + B_AMB on the ghost planes,
                every other variable as with B_AMB = 0
-The last tests pin the finding that the reference's hll_d_flux_dir ("hlld_full") diverges on
-the 256^3 RMHD jet's setup within two steps (at 32^3), while HLLE stays bounded.
+RESISTIVE_MODE "once" (the 512^3 jet applies the resistive Laplacian once in WENO, where the
+reference applies it never) is pinned by the Laplacian it adds.  The remaining tests pin the
+finding that the reference's hll_d_flux_dir ("hlld_full") diverges on the 256^3 RMHD jet's setup
+within two steps (at 32^3), while HLLE stays bounded.
 """
 import os
 
@@ -129,3 +131,30 @@ def test_hlle_stays_bounded_on_rmhd_jet():
     assert rho.max() < 3.0
     floored = np.argwhere(rho < 1e-3)
     assert len(floored) <= 3 and np.all(floored[:, 0] == 0)
+
+
+@pytest.mark.parametrize("recon", ["weno", "ppm", "muscl"])
+def test_resistive_once_adds_one_laplacian(recon):
+    """RESISTIVE_MODE "once" (the 512^3 jet's key): the RHS exceeds the ideal one by
+    eta * Lap(B) in the B components, with the reference's 7-point Laplacian and 1 / (d^2 + SMALL)
+    weights (core/rmhd_core.py:1265-1275), once in every scheme, and not at all elsewhere."""
+    from tests.golden_states import smooth_state
+    ng, eta = 3, 1e-2
+    pr = smooth_state((14, 15, 16), 21, "rmhd", vamp=0.2)
+    dx, dy, dz = 1.0 / 8, 1.0 / 9, 1.0 / 10
+    kw = dict(ng=ng, gamma=4.0 / 3.0, p_max=2.0, v_max=0.9999, recon=recon, riemann="hlle")
+    ideal = O.rhs(O.make_params("rmhd", pr.shape[1:], **kw), pr, dx, dy, dz)
+    once = O.rhs(O.make_params("rmhd", pr.shape[1:], resistive=True, resistivity=eta, resist_once=True, **kw),
+                 pr, dx, dy, dz)
+    lo = 2 if recon == "muscl" else 3
+    hi = [n - (2 if recon == "muscl" else 3) for n in pr.shape[1:]]
+    sl = (slice(lo, hi[0]), slice(lo, hi[1]), slice(lo, hi[2]))
+    w = [1.0 / (h * h + 1e-12) for h in (dx, dy, dz)]
+    for c in range(5, 8):
+        q = pr[c]
+        lap = ((np.roll(q, -1, 0) - 2 * q + np.roll(q, 1, 0)) * w[0] + (np.roll(q, -1, 1) - 2 * q + np.roll(q, 1, 1)) * w[1]
+               + (np.roll(q, -1, 2) - 2 * q + np.roll(q, 1, 2)) * w[2])
+        d = once[c][sl] - ideal[c][sl]
+        assert np.allclose(d, eta * lap[sl], rtol=1e-9, atol=1e-12 * np.abs(ideal[c]).max()), c
+    for v in (0, 1, 2, 3, 4, 8):
+        assert np.array_equal(once[v], ideal[v]), v
