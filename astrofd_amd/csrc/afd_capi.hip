@@ -33,7 +33,9 @@ struct afd_ctx {
     Grid g{};
     int nb = 5;
     bool carried = false, passive_flux = false;
-    bool dmode = false;  // mixed RMHD WENO5 with shared edges: the faces accumulate the flux divergence in Fx
+    bool shared_edges = false;  // mixed RMHD WENO5: the shared-edge face kernels
+    bool dmode = false;  // ... accumulating the flux divergence in Fx
+    bool fused = false;  // ... with the y faces evaluated by the stage update
     int diss = 0;  // FaceArgs::diss of the SRHD faces
     double *Puse = nullptr;  // GR ORTHONORMAL_FLUX: the stage input with rescaled velocities
     size_t fsize = 8;
@@ -207,7 +209,7 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
     if (p.n_resist < 0 || p.n_resist > 2) return fail(AFD_ERR_INVALID, "n_resist must be 0, 1 or 2");
     if (p.gr_metric < 0 || p.gr_metric > 3) return fail(AFD_ERR_INVALID, "gr_metric must be in [0, 3]");
     if (p.gr_metric != 0 && !(p.gr_gamma > 1.0)) return fail(AFD_ERR_INVALID, "gr_gamma must be > 1");
-    if (p.face_path != 0 && p.face_path != 1) return fail(AFD_ERR_INVALID, "face_path must be 0 or 1");
+    if (p.face_path < 0 || p.face_path > 4) return fail(AFD_ERR_INVALID, "face_path must be in [0, 4]");
 
     HIPCHK(hipSetDevice(p.device));
     afd_ctx *c = new afd_ctx();
@@ -241,8 +243,14 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
     c->carried = nb == 9 && p.recon_id != AFD_RECON_MUSCL;
     c->passive_flux = p.n_passive > 0 && (nb == 9 ? p.recon_id == AFD_RECON_MUSCL : p.recon_id != AFD_RECON_WENO);
     c->fsize = p.precision == AFD_PREC_MIXED ? 4 : 8;
-    c->dmode = nb == 9 && p.precision == AFD_PREC_MIXED && p.recon_id == AFD_RECON_WENO && p.face_path == 0 &&
-               g.ncell < (1L << 29);
+    // the mixed RMHD WENO5 face variants (afd_params.face_path): shared edges below 2^29 cells, with the
+    // face fluxes stored (2) or the flux divergence accumulated (3; 4: with the y faces in the update)
+    const int fp = p.face_path == 0 ? AFD_FACE_PATH_DEFAULT : p.face_path;
+    const bool shared = nb == 9 && p.precision == AFD_PREC_MIXED && p.recon_id == AFD_RECON_WENO && fp >= 2 &&
+                        g.ncell < (1L << 29);
+    c->shared_edges = shared;
+    c->dmode = shared && fp >= 3;
+    c->fused = shared && fp == 4;
     if (nb == 5) {  // core/srhd_core.py:393-462: bulk pressure (MUSCL), dissipative fluxes (MUSCL 1x, PPM 2x)
         int reps = p.nv >= 15 ? (p.recon_id == AFD_RECON_MUSCL ? 1 : (p.recon_id == AFD_RECON_PPM ? 2 : 0)) : 0;
         c->diss = (p.recon_id == AFD_RECON_MUSCL && p.passive_offset > 5 ? 1 : 0) | (reps << 1);
@@ -284,7 +292,7 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
     if (nb == 9 && p.c2p_guess != AFD_C2P_PLAIN) {
         long blocks = (long)grid1d(c->g.ncell).x;
         c->c2p_qcap = (blocks + QUEUE_SLOTS - 1) / QUEUE_SLOTS * BLOCK;
-        if (c->dmode) {  // k_face_march_update (up to 64 x MARCH_CELLS cells a workgroup) + k_update_shell
+        if (c->fused) {  // k_face_march_update (up to 64 x MARCH_CELLS cells a workgroup) + k_update_shell
             long fused = (fused_blocks(c->g) + QUEUE_SLOTS - 1) / QUEUE_SLOTS * 64L * MARCH_CELLS;
             long shell = ((long)grid1d(shell_cells(c->g)).x + QUEUE_SLOTS - 1) / QUEUE_SLOTS * BLOCK;
             c->c2p_qcap = std::max(c->c2p_qcap, fused + shell);
@@ -553,7 +561,7 @@ static FaceLaunch face_launch(afd_ctx *c, const double *P) {
     l.dir = 0;
     l.diss = c->diss;
     l.mixed = c->p.precision == AFD_PREC_MIXED;
-    l.shared_edges = c->p.face_path == 0;
+    l.shared_edges = c->shared_edges;
     l.D = c->dmode ? (float *)c->Fx : nullptr;
     l.idn = 0.0f;
     return l;
@@ -664,12 +672,12 @@ extern "C" int afd_step(afd_ctx *c, double dt) {
     int src = in;
     for (int s = 0; s < nst; ++s) {
         int dst = (s % 2 == 0) ? a : b;
-        int rc = run_faces(c, c->P[src], c->dmode);
+        int rc = run_faces(c, c->P[src], c->fused);
         if (rc) return rc;
         UpdLaunch l = upd_launch(c, c->P[src], c->P[in], c->P[dst], s, nst, dt);
         Timed t(c, 3);
         hipError_t e;
-        if (c->dmode) {  // the y faces with the stage update
+        if (c->fused) {  // the y faces with the stage update
             FaceLaunch fl = face_launch(c, c->Puse ? c->Puse : c->P[src]);
             fl.dir = 1;
             fl.idn = (float)(1.0 / c->p.dy);

@@ -84,44 +84,54 @@ template <typename T> struct FaceArgs {
 
 // ------------------------------------------------------------------ reconstruction
 // Face states of the face between cells c-1 and c along stride s, variable base pointer b.
-// The mixed path reconstructs deviations from q[c] in FP32 and adds them back in FP64.  Returns
-// the smaller WENO5 weight sum of the FP32 path (1 otherwise); below WENO_SUM_MIN the caller
-// evaluates the face again with RESCUE set (the rescaled FP32 WENO5).
+// The mixed path reconstructs in FP32 each edge as a deviation from its own cell's value (the
+// left state from q[c-1], the right state from q[c]) and adds it back in FP64, so that across a
+// large jump neither state is the small difference of two large values.  Returns the smaller
+// WENO5 weight sum of the FP32 path (1 otherwise); below WENO_SUM_MIN the caller evaluates the
+// face again with RESCUE set (the rescaled FP32 WENO5).
 template <int RECON, typename T, bool RESCUE = false>
 AFD_D float recon_face(const double *__restrict__ b, long c, long s, int lim, double &qL, double &qR) {
     constexpr bool mixed = sizeof(T) == 4;
-    const double ref = mixed ? b[c] : 0.0;
+    const double rL = mixed ? b[c - s] : 0.0, rR = mixed ? b[c] : 0.0;
     float smin = 1.0f;
     T l, r;
     if (RECON == 0) {  // MUSCL: core/srhd_core.py:486-533
-        T qm2 = (T)(b[c - 2 * s] - ref), qm1 = (T)(b[c - s] - ref), q0 = (T)(b[c] - ref), qp1 = (T)(b[c + s] - ref);
-        T slL = limiter<T>(lim, qm1 - qm2, q0 - qm1);
-        T slR = limiter<T>(lim, q0 - qm1, qp1 - q0);
-        l = qm1 + T(0.5) * slL;
-        r = q0 - T(0.5) * slR;
+        const double bm2 = b[c - 2 * s], bm1 = b[c - s], b0 = b[c], bp1 = b[c + s];
+        {  // left state: cell c-1 and its neighbours
+            T qm2 = (T)(bm2 - rL), qm1 = (T)(bm1 - rL), q0 = (T)(b0 - rL);
+            T slL = limiter<T>(lim, qm1 - qm2, q0 - qm1);
+            l = qm1 + T(0.5) * slL;
+        }
+        {  // right state: cell c and its neighbours
+            T qm1 = (T)(bm1 - rR), q0 = (T)(b0 - rR), qp1 = (T)(bp1 - rR);
+            T slR = limiter<T>(lim, q0 - qm1, qp1 - q0);
+            r = q0 - T(0.5) * slR;
+        }
     } else {
-        T qm3 = (T)(b[c - 3 * s] - ref), qm2 = (T)(b[c - 2 * s] - ref), qm1 = (T)(b[c - s] - ref);
-        T q0 = (T)(b[c] - ref), qp1 = (T)(b[c + s] - ref), qp2 = (T)(b[c + 2 * s] - ref);
+        const double bm3 = b[c - 3 * s], bm2 = b[c - 2 * s], bm1 = b[c - s], b0 = b[c], bp1 = b[c + s],
+                     bp2 = b[c + 2 * s];
+        T lm3 = (T)(bm3 - rL), lm2 = (T)(bm2 - rL), lm1 = (T)(bm1 - rL), l0 = (T)(b0 - rL), lp1 = (T)(bp1 - rL);
+        T um2 = (T)(bm2 - rR), um1 = (T)(bm1 - rR), u0 = (T)(b0 - rR), up1 = (T)(bp1 - rR), up2 = (T)(bp2 - rR);
         if (RECON == 1) {  // PPM: core/srhd_core.py:943-950
             T a, bb;
-            ppm_reconstruct<T>(qm3, qm2, qm1, q0, qp1, a, l);
-            ppm_reconstruct<T>(qm2, qm1, q0, qp1, qp2, r, bb);
+            ppm_reconstruct<T>(lm3, lm2, lm1, l0, lp1, a, l);
+            ppm_reconstruct<T>(um2, um1, u0, up1, up2, r, bb);
         } else if constexpr (!mixed) {  // WENO5: core/srhd_core.py:1183-1184
-            l = weno5_left<T>(qm3, qm2, qm1, q0, qp1);
-            r = weno5_right<T>(qp2, qp1, q0, qm1, qm2);
+            l = weno5_left<T>(lm3, lm2, lm1, l0, lp1);
+            r = weno5_right<T>(up2, up1, u0, um1, um2);
         } else if constexpr (RESCUE) {
-            l = weno5_left_f32_rescaled(qm3, qm2, qm1, q0, qp1);
-            r = weno5_left_f32_rescaled(qp2, qp1, q0, qm1, qm2);
+            l = weno5_left_f32_rescaled(lm3, lm2, lm1, l0, lp1);
+            r = weno5_left_f32_rescaled(up2, up1, u0, um1, um2);
         } else {
             float s1, s2;
-            l = weno5_left_f32_eps(qm3, qm2, qm1, q0, qp1, 1e-12f, s1);
-            r = weno5_left_f32_eps(qp2, qp1, q0, qm1, qm2, 1e-12f, s2);
+            l = weno5_left_f32_eps(lm3, lm2, lm1, l0, lp1, 1e-12f, s1);
+            r = weno5_left_f32_eps(up2, up1, u0, um1, um2, 1e-12f, s2);
             smin = fminf(s1, s2);
         }
     }
     if (mixed) {
-        qL = ref + (double)l;
-        qR = ref + (double)r;
+        qL = rL + (double)l;
+        qR = rR + (double)r;
     } else {
         qL = (double)l;
         qR = (double)r;

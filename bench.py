@@ -240,26 +240,44 @@ def measured_traffic(workload):
     return doc.get("traffic_bytes_per_launch", {}), src
 
 
-def flux_divergence_path(s, mixed):
-    """The mixed RMHD WENO5 path on grids below 2^29 cells: the face kernels accumulate the flux
-    divergence (astrofd_amd/csrc/afd_kernels.hpp, k_face_rmhd_weno_lanes_div / _march)."""
+def face_variant(s, mixed, face_path=0):
+    """The mixed RMHD WENO5 face-kernel variant a run takes (afd_params.face_path; 0 selects
+    AFD_FACE_PATH_DEFAULT): 1 one thread per face, 2 shared edges with stored face fluxes,
+    3 flux divergence, 4 flux divergence with the y faces in the stage update.  0 for every other
+    workload (one thread per face)."""
+    from astrofd_amd import _lib as L
     from astrofd_amd.domain import recon_id
     cells = (s["NX"] + 2 * s["NG"]) * (s["NY"] + 2 * s["NG"]) * (s["NZ"] + 2 * s["NG"])
-    return s["PHYSICS"] == "rmhd" and mixed and recon_id(s["RECON"]) == 2 and cells < (1 << 29)
+    if not (s["PHYSICS"] == "rmhd" and mixed and recon_id(s["RECON"]) == 2):
+        return 0
+    v = face_path or L.AFD_FACE_PATH_DEFAULT
+    return v if v == 1 or cells < (1 << 29) else 1
 
 
-def kernel_name(s, which, mixed, learned):
+def flux_divergence_path(s, mixed, face_path=0):
+    """The face kernels accumulate the flux divergence (astrofd_amd/csrc/afd_kernels.hpp,
+    k_face_rmhd_weno_lanes_div / _march) instead of storing face fluxes."""
+    return face_variant(s, mixed, face_path) >= 3
+
+
+def kernel_name(s, which, mixed, learned, face_path=0):
     """The rocprofv3 name of a kernel of this workload (face_x/y/z or stage_update_c2p)."""
     T = "float" if mixed else "double"
     nb = 9 if s["PHYSICS"] == "rmhd" else 5
-    if which == "stage_update_c2p":
-        return f"k_update<{nb}, {T}, {'true' if learned else 'false'}>"
+    v = face_variant(s, mixed, face_path)
     from astrofd_amd.domain import recon_id, riemann_id
-    d = {"face_x": 0, "face_y": 1, "face_z": 2}[which]
     rid = riemann_id(s['PHYSICS'], s['RIEMANN'])
-    if flux_divergence_path(s, mixed):
+    if which == "stage_update_c2p":
+        if v == 4:
+            return f"k_face_march_update<{rid}, {'true' if learned else 'false'}>"
+        return f"k_update<{nb}, {T}, {'true' if learned else 'false'}>"
+    d = {"face_x": 0, "face_y": 1, "face_z": 2}[which]
+    if v >= 3:
         # FP32 WENO5 faces accumulating the flux divergence (pencils for x/y, lanes for z)
         return f"k_face_rmhd_weno_lanes_div<{rid}>" if d == 2 else f"k_face_rmhd_weno_march<{rid}, {d}>"
+    if v == 2:
+        # FP32 WENO5 faces sharing edge values (LDS rows for x/y, lanes for z), fluxes stored
+        return f"k_face_rmhd_weno_lanes<{rid}>" if d == 2 else f"k_face_rmhd_weno_rows<{rid}, {d}>"
     kind = "k_face_rmhd" if nb == 9 else "k_face_srhd"
     return f"{kind}<{recon_id(s['RECON'])}, {rid}, {d}, {T}>"
 
@@ -277,7 +295,7 @@ def algorithmic_update_bytes(s):
     return (13 if nst == 3 else 8) * nvars(s) * 8 / nst
 
 
-def impl_update_bytes(s, mixed):
+def impl_update_bytes(s, mixed, face_path=0):
     """HBM bytes per cell of one stage update as this implementation moves them, averaged over the
     stages: read P0 (U0 = p2c(P0) every stage), read U_s-1 (stages > 0), write U_s (not the last
     stage), write P_s, read psi and the passives of P_s-1, and the face terms: one FP32 flux
@@ -287,7 +305,7 @@ def impl_update_bytes(s, mixed):
     nst = 3 if s["RK_ORDER"] == 3 else 2
     fb = 4 if mixed else 8
     carried = nb == 9 and s["RECON"] != "muscl"
-    face = nb * 4 if flux_divergence_path(s, mixed) else (4 if carried else 3) * nb * fb
+    face = nb * 4 if flux_divergence_path(s, mixed, face_path) else (4 if carried else 3) * nb * fb
     per = []
     for st in range(nst):
         b = nv * 8 + nv * 8 + face + (8 if nb == 9 else 0) + (nv - nb) * 8
@@ -297,7 +315,7 @@ def impl_update_bytes(s, mixed):
     return sum(per) / nst
 
 
-def face_bytes(s, mixed, dkey):
+def face_bytes(s, mixed, dkey, face_path=0):
     """Compulsory bytes per face (flux path) or per cell (flux-divergence path) of a face kernel:
     read the nb primitives of one cell (the stencil's other cells are re-reads) and write its output:
     the face flux (twice for the RMHD PPM/WENO x faces, which also read the two carried speeds the
@@ -305,7 +323,7 @@ def face_bytes(s, mixed, dkey):
     nb = 9 if s["PHYSICS"] == "rmhd" else 5
     fb = 4 if mixed else 8
     carried = nb == 9 and s["RECON"] != "muscl"
-    if flux_divergence_path(s, mixed):
+    if flux_divergence_path(s, mixed, face_path):
         return [nb * 8 + 2 * 4 + 2 * nb * 4, nb * 8 + 2 * nb * 4, nb * 8 + nb * 4 + 2 * 4][dkey]
     b = nb * 8 + nb * fb
     if carried and dkey == 0:
@@ -315,7 +333,7 @@ def face_bytes(s, mixed, dkey):
     return b
 
 
-def kernel_roofline(s, kern, mixed, learned=False, workload=None):
+def kernel_roofline(s, kern, mixed, learned=False, workload=None, face_path=0):
     """Dominant kernel and its achieved algorithmic HBM bandwidth (bytes per launch / avg duration).
 
     Stage update (one thread per cell, ghosts included): the metric's compulsory state traffic per
@@ -327,20 +345,20 @@ def kernel_roofline(s, kern, mixed, learned=False, workload=None):
     NX, NY, NZ = s["NX"], s["NY"], s["NZ"]
     full = (NX + 2 * s["NG"]) * (NY + 2 * s["NG"]) * (NZ + 2 * s["NG"])
     units = [(NX + 1) * NY * NZ, NX * (NY + 1) * NZ, NX * NY * (NZ + 1)]
-    if flux_divergence_path(s, mixed):
+    if flux_divergence_path(s, mixed, face_path):
         units = [NX * NY * NZ] * 3
     cands = []
     for dkey, name in ((0, "face_x"), (1, "face_y"), (2, "face_z")):
         n = kern["launches_face"][dkey]
         if n:
-            nbytes = face_bytes(s, mixed, dkey) * units[dkey]
+            nbytes = face_bytes(s, mixed, dkey, face_path) * units[dkey]
             cands.append((kern["ms_face"][dkey] / n, name, nbytes, nbytes))
     if kern["launches_update"]:
         cands.append((kern["ms_update"] / kern["launches_update"], "stage_update_c2p",
-                      algorithmic_update_bytes(s) * full, impl_update_bytes(s, mixed) * full))
+                      algorithmic_update_bytes(s) * full, impl_update_bytes(s, mixed, face_path) * full))
     ms, name, nbytes, impl = max(cands)
     achieved = nbytes / (ms * 1e-3) / 1e9
-    kname = kernel_name(s, name, mixed, learned)
+    kname = kernel_name(s, name, mixed, learned, face_path)
     table, src = measured_traffic(workload) if workload else ({}, None)
     traffic = table.get(kname)
     return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -571,6 +589,8 @@ def main():
     ap.add_argument("--accuracy-steps", default="1,2,5,10")
     ap.add_argument("--face-path-ab", action="store_true",
                     help="also time the mixed run with one thread per face (afd_params.face_path = 1)")
+    ap.add_argument("--face-path-sweep", action="store_true",
+                    help="time the mixed run with every face-kernel variant (afd_params.face_path 1..4)")
     ap.add_argument("--evolve", action="store_true",
                     help="time consecutive steps of one run instead of steps from the seeded state")
     ap.add_argument("--evolve-steps", type=int, default=10,
@@ -633,6 +653,15 @@ def main():
         mixed_pf = run_timed(sm, pr0, a.steps, a.warmup, mlp=mlp, repeat=not a.evolve, face_path=1)
         log(f"[bench]   {mixed_pf['elapsed'] * 1e3 / a.steps:.2f} ms/step")
 
+    if a.face_path_sweep:
+        sweep = {}
+        for fp in (1, 2, 3, 4):
+            log(f"[bench] mixed run with face_path = {fp}")
+            r = run_timed(sm, pr0, a.steps, a.warmup, mlp=mlp, repeat=not a.evolve, face_path=fp)
+            sweep[str(fp)] = {"ms_per_step": round(r["elapsed"] * 1e3 / a.steps, 3),
+                              "kernel_ms_per_step": kernel_ms(r["kernels"], a.steps)}
+            log(f"[bench]   {sweep[str(fp)]}")
+
     value = zu / mixed["elapsed"]
     nb = 9 if s["PHYSICS"] == "rmhd" else 5
     rm = s["PHYSICS"] == "rmhd"
@@ -666,6 +695,8 @@ def main():
                                        "ms_per_step": round(mixed_prev["elapsed"] * 1e3 / a.steps, 3),
                                        "kernel_ms_per_step": kernel_ms(mixed_prev["kernels"], a.steps),
                                        "c2p": c2p_report(mixed_prev["stats"])}
+    if a.face_path_sweep:
+        res["face_path_sweep"] = sweep
     if mixed_pf is not None:
         res["mixed_per_face_kernels"] = {"value": round(zu / mixed_pf["elapsed"], 1),
                                          "ms_per_step": round(mixed_pf["elapsed"] * 1e3 / a.steps, 3),

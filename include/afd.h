@@ -77,10 +77,16 @@ typedef struct afd_params {
     int32_t gr_x0;            /* global x index of the first interior cell (0 on one GPU) */
     double gr_mass;           /* GR_MASS */
     double gr_gamma;          /* Gamma of the sources' enthalpy (GRHD: the configured GAMMA) */
-    int32_t face_path;        /* 0: automatic (the FP32 RMHD WENO5 faces share cell edge values on grids
-                                 below 2^29 cells); 1: one thread per face everywhere.  Same results to
-                                 FP32 rounding; no counterpart in the reference */
+    int32_t face_path;        /* mixed RMHD WENO5 face kernels (no counterpart in the reference; all give the
+                                 same results to FP32 rounding): 0 automatic; 1 one thread per face;
+                                 2 cells share their edge values (LDS rows, lane shuffles), face fluxes
+                                 stored; 3 as 2, the faces accumulate each cell's flux divergence instead;
+                                 4 as 3, with the y faces evaluated by the stage update.  2-4 apply below
+                                 2^29 cells, larger grids take 1 */
 } afd_params;
+
+/* The variant face_path 0 selects (the fastest measured on the 256^3 RMHD WENO5 step). */
+#define AFD_FACE_PATH_DEFAULT 2
 
 /* Nozzle inflow on the -x ghost planes (core/nozzle.py:27-92).  The library's host code
  * evaluates nozzle_weight (core/nozzle.py:5-25) per (j, k) and uploads the profile. */

@@ -28,6 +28,13 @@ def test_binding_covers_header():
     assert set(declared_symbols()) <= set(L.exported_symbols())
 
 
+def test_binding_constants_match_header():
+    text = open(os.path.join(ROOT, "include", "afd.h")).read()
+    for name in ("AFD_FACE_PATH_DEFAULT",):
+        m = re.search(rf"^#define {name} (\d+)", text, flags=re.M)
+        assert m and int(m.group(1)) == getattr(L, name), name
+
+
 def test_version_and_param_count_without_device():
     lib = L.load()
     assert b"gfx950" in lib.afd_version()
@@ -74,7 +81,7 @@ def test_parameter_validation_without_a_device():
         (dict(gr_metric=1, gr_gamma=1.0), b"gr_gamma"),
         (dict(nv=40, n_passive=35), b"nv > 32"),
         (dict(nx=1285, ny=1285, nz=1285), b"grid too large"),  # 1291^3 >= 2^31 cells with ghosts
-        (dict(face_path=2), b"face_path"),
+        (dict(face_path=5), b"face_path"),
     ]
     for kw, msg in cases:
         h = ctypes.c_void_p()
