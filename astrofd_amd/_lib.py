@@ -18,6 +18,12 @@ AFD_C2P_PLAIN, AFD_C2P_LEARNED, AFD_C2P_PREVIOUS = 0, 1, 2
 AFD_BC_PERIODIC, AFD_BC_OUTFLOW = 0, 1
 AFD_OK, AFD_ERR_INVALID, AFD_ERR_NOMEM, AFD_ERR_HIP, AFD_ERR_STATE = 0, -1, -2, -3, -4
 AFD_FACE_PATH_DEFAULT = 2  # the face-kernel variant afd_params.face_path = 0 selects
+AFD_NOALLOC_SCRATCH_BYTES = 16 << 20  # device staging of afd_c2p_batch_noalloc, per device
+
+
+def c2p_noalloc_max(nb):
+    """AFD_C2P_NOALLOC_MAX(nb): states one afd_c2p_batch_noalloc call can convert."""
+    return AFD_NOALLOC_SCRATCH_BYTES // (2 * 8 * nb + 4)
 
 _i32, _i64, _dbl, _f32 = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_float
 _D = ctypes.POINTER(ctypes.c_double)
@@ -118,6 +124,7 @@ SIGNATURES = {
     "afd_count_nonfinite": [_VP, ctypes.POINTER(_i64)],
     "afd_p2c_batch": [_VP, _i64, _D, _D],
     "afd_c2p_batch": [_VP, _i64, _D, _D, ctypes.POINTER(_i32)],
+    "afd_c2p_batch_noalloc": [_VP, _i64, _D, _D, ctypes.POINTER(_i32)],
     "afd_diagnostics": [_VP, ctypes.POINTER(Diag)],
     "afd_centerline": [_VP, _D, _i64],
     "afd_set_instrumentation": [_VP, _i32, _i32],

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -16,6 +17,7 @@
 using namespace afd;
 
 static thread_local std::string g_err;
+static int resolve_scratch(int device);
 
 static int fail(int code, const std::string &msg) {
     g_err = msg;
@@ -212,6 +214,7 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
     if (p.face_path < 0 || p.face_path > 4) return fail(AFD_ERR_INVALID, "face_path must be in [0, 4]");
 
     HIPCHK(hipSetDevice(p.device));
+    if (int rc0 = resolve_scratch(p.device)) return rc0;  // afd_c2p_batch_noalloc's per-device staging
     afd_ctx *c = new afd_ctx();
     c->p = p;
     c->nb = nb;
@@ -916,6 +919,64 @@ extern "C" int afd_p2c_batch(afd_ctx *c, int64_t n, const double *prim, double *
 }
 extern "C" int afd_c2p_batch(afd_ctx *c, int64_t n, const double *cons, double *prim, int32_t *iters) {
     return batch(c, n, cons, prim, iters, true);
+}
+
+// ------------------------------------------------------------------ allocation-free con2prim batch
+// Device staging of afd_c2p_batch_noalloc: the states in, the states out and the iteration
+// counts, AFD_NOALLOC_SCRATCH_BYTES per device for the life of the process.  Host code never
+// uses this name as an address: the per-device address comes from hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[AFD_NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device (set-up of afd_create, once per device).
+static int resolve_scratch(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        return fail(AFD_ERR_INVALID, "device ordinal outside [0, 64) for the allocation-free entry point");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return AFD_OK;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(AFD_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+    return AFD_OK;
+}
+
+static void *scratch_ptr(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return (device >= 0 && device < NOALLOC_MAX_DEVICES) ? g_scratch_ptr[device] : nullptr;
+}
+
+extern "C" int afd_c2p_batch_noalloc(afd_ctx *c, int64_t n, const double *cons, double *prim, int32_t *iters) {
+    if (!c || (n > 0 && (!cons || !prim))) return fail(AFD_ERR_INVALID, "afd_c2p_batch_noalloc: null argument");
+    if (n < 0) return fail(AFD_ERR_INVALID, "afd_c2p_batch_noalloc: n < 0");
+    const int64_t nmax = AFD_C2P_NOALLOC_MAX(c->nb);
+    if (n > nmax)
+        return fail(AFD_ERR_INVALID, "afd_c2p_batch_noalloc: n = " + std::to_string(n) + " exceeds the " +
+                                         std::to_string(nmax) + " states the " +
+                                         std::to_string(AFD_NOALLOC_SCRATCH_BYTES) + "-byte device scratch holds");
+    if (n == 0) return AFD_OK;
+    HIPCHK(hipSetDevice(c->p.device));
+    if (int rc = resolve_scratch(c->p.device)) return rc;  // set up by afd_create; kept for safety
+    char *base = (char *)scratch_ptr(c->p.device);
+    const size_t bytes = (size_t)n * c->nb * 8;
+    double *din = (double *)base, *dout = (double *)(base + bytes);
+    int *dit = iters ? (int *)(base + 2 * bytes) : nullptr;
+    HIPCHK(hipMemcpyAsync(din, cons, bytes, hipMemcpyHostToDevice, c->stream));
+    BatchArgs a{din,           dout,         dit,        c->stats_on ? c->stats_dev : nullptr,
+                (long)n,       c->p.physics, c->p.gamma, c->p.gamma / (c->p.gamma - 1.0),
+                c->p.p_max,    c->p.v_max};
+    if (c->stats_on) c->acc.c2p_cells += (uint64_t)n;
+    C2PBatchExtra x{c->p.c2p_guess == AFD_C2P_LEARNED && c->mlp.loaded ? c->mlp.w : nullptr};
+    launch_c2p_batch(a, x, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(prim, dout, bytes, hipMemcpyDeviceToHost, c->stream));
+    if (iters) HIPCHK(hipMemcpyAsync(iters, dit, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AFD_OK;
 }
 
 // ------------------------------------------------------------------ diagnostics

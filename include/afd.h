@@ -236,6 +236,18 @@ int afd_count_nonfinite(afd_ctx *ctx, int64_t *count);
 int afd_p2c_batch(afd_ctx *ctx, int64_t n, const double *prim, double *cons);
 int afd_c2p_batch(afd_ctx *ctx, int64_t n, const double *cons, double *prim, int32_t *iters);
 
+/* afd_c2p_batch without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Same kernels and results as afd_c2p_batch; the device
+ * staging of the n states in, the n states out and the n iteration counts lives in a
+ * library-owned static buffer of AFD_NOALLOC_SCRATCH_BYTES on each device the library runs on,
+ * held until the process ends (its address is resolved by afd_create).  n above
+ * AFD_C2P_NOALLOC_MAX(nb) (nb = 5 SRHD, 9 RMHD) returns AFD_ERR_INVALID.  Every call on a device
+ * shares that buffer: calls that overlap on one GPU (several contexts or threads) race. */
+#define AFD_NOALLOC_SCRATCH_BYTES (16u << 20)
+#define AFD_C2P_NOALLOC_MAX(nb) ((int64_t)(AFD_NOALLOC_SCRATCH_BYTES / (2u * 8u * (unsigned)(nb) + 4u)))
+int afd_c2p_batch_noalloc(afd_ctx *ctx, int64_t n, const double *cons, double *prim, int32_t *iters);
+
 /* utils/diagnostics.py:5-57 (max Gamma, inlet flux) and :128-183 (div B), as device reductions. */
 int afd_diagnostics(afd_ctx *ctx, afd_diag *out);
 /* utils/diagnostics.py:59-126: the mid-plane centerline, out = 4 x nx (Gamma, rho, p, vx). */
