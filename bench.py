@@ -24,7 +24,8 @@ step, the fused post-step source kernel (cooling, two-temperature, chemistry)
 on the same grid, and the CPU oracle's rate on a bounded sample on this host's
 cores.  Beside the headline:
   fp64_ulp_spread  the plain FP64 step's own response to a 1-ulp perturbation of its input
-                   (the reference's hll_d_flux_dir is ill-conditioned over a static ambient)
+                   (the reference's hll_d_flux_dir is ill-conditioned over a static ambient);
+                   every errors entry carries the same spread after its own step
   hlle_50          50 consecutive steps of the same jet with RIEMANN = "hlle" (the reference's
                    own WENO RMHD choice, stable on this jet): mixed + learned against plain FP64
   network          the learned guess's MLP alone over 2^24 samples: achieved BF16 MFMA rate
@@ -169,16 +170,25 @@ def run_timed(s, pr0, steps, warmup, mlp=None, repeat=True, face_path=0):
         d.close()
 
 
+def ulp_perturbed(pr0):
+    """The state with every value moved by 1 ulp (seeded signs)."""
+    return pr0 * (1.0 + 2.220446049250313e-16 * np.random.default_rng(99).choice([-1.0, 1.0], pr0.shape))
+
+
 def accuracy(sp, sm, pr0, checkpoints, mlp=None, flips=False):
     """Plain and mixed runs in lockstep from the same state; errors of mixed against plain after
-    each checkpoint step (cells where both are finite), and the non-finite counts.  flips: also
-    count the cells whose last-stage con2prim took the other branch of the hydro fixed point
+    each checkpoint step (cells where both are finite), and the non-finite counts.  A third run,
+    plain FP64 from the state perturbed by 1 ulp, gives the scheme's own sensitivity at the same
+    steps (fp64_ulp_spread: its errors against the plain run): an implementation that rounds
+    differently from the plain run cannot be expected to stay closer to it than that.  flips:
+    also count the cells whose last-stage con2prim took the other branch of the hydro fixed point
     (converged vs energy fallback) in the two runs, at each checkpoint step."""
     dp, rp = open_domain(sp, pr0)
     dm, rm = open_domain(sm, pr0, mlp)
+    dq, rq = open_domain(sp, ulp_perturbed(pr0))
     out = []
     try:
-        step_p, step_m = make_stepper(dp, sp, rp), make_stepper(dm, sm, rm)
+        step_p, step_m, step_q = make_stepper(dp, sp, rp), make_stepper(dm, sm, rm), make_stepper(dq, sp, rq)
         for n in range(1, max(checkpoints) + 1):
             cp = n in checkpoints
             if cp and flips:
@@ -186,13 +196,18 @@ def accuracy(sp, sm, pr0, checkpoints, mlp=None, flips=False):
                 dm.set_instrumentation(stats=True, timing=False)
             step_p()
             step_m()
+            step_q()
             if cp:
                 a, b = dm.get_prim(), dp.get_prim()
                 e = errors(a, b, sp["NG"])
                 e["step"] = n
                 e["nonfinite"] = {"plain": int(np.count_nonzero(~np.isfinite(b))),
                                   "mixed": int(np.count_nonzero(~np.isfinite(a)))}
-                del a, b
+                del a
+                q = dq.get_prim()
+                e["fp64_ulp_spread"] = {k: {"L1_rel": v["L1_rel"], "max_abs": v["max_abs"]}
+                                        for k, v in errors(q, b, sp["NG"]).items()}
+                del q, b
                 if flips:
                     ng = sp["NG"]
                     sl = (slice(ng, -ng),) * 3
@@ -207,24 +222,8 @@ def accuracy(sp, sm, pr0, checkpoints, mlp=None, flips=False):
     finally:
         dp.close()
         dm.close()
+        dq.close()
     return out
-
-
-def ulp_spread(sp, pr0):
-    """The plain FP64 step's response to its own rounding: one step from the seeded state and one
-    from the same state perturbed by 1 ulp (seeded signs), errors of the second against the first."""
-    pert = pr0 * (1.0 + 2.220446049250313e-16 * np.random.default_rng(99).choice([-1.0, 1.0], pr0.shape))
-    res = []
-    for init in (pr0, pert):
-        d, rng = open_domain(sp, init)
-        try:
-            make_stepper(d, sp, rng)()
-            res.append(d.get_prim())
-        finally:
-            d.close()
-    e = errors(res[1], res[0], sp["NG"])
-    e["step"] = 1
-    return e
 
 
 def measured_traffic(workload):
@@ -462,7 +461,7 @@ BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
 
 
 def network_rate(mlp, n=1 << 24, launches=3):
-    """The learned guess's MLP (5 -> 64 -> 64 -> 2, SiLU, BF16 weights, FP32 accumulation) alone over
+    """The learned guess's MLP (5 -> 64 -> 64 -> 2, ReLU, BF16 weights, FP32 accumulation) alone over
     n samples: kernel time from HIP events on the library's stream, achieved FLOP rate against the
     dense BF16 MFMA peak.  The inputs are uniform in [-1, 1] (the rate does not depend on them)."""
     from astrofd_amd.domain import Domain
@@ -716,8 +715,7 @@ def main():
             # show how the reference scheme's own divergence (non-finite counts) takes over
             res["errors_vs_fp64_plain"] = ev[0]
             res["errors_evolution"] = ev
-            log("[bench] FP64 response to 1-ulp input noise (one step)")
-            res["fp64_ulp_spread"] = ulp_spread(sp, pr0)
+            res["fp64_ulp_spread"] = dict(ev[0]["fp64_ulp_spread"], step=1)
         if a.evolve_steps > 0 and not a.evolve:
             zu_e = s["NX"] * s["NY"] * s["NZ"] * a.evolve_steps
             log(f"[bench] {a.evolve_steps} consecutive steps (plain, mixed)")
