@@ -17,7 +17,7 @@ AFD_PREC_FP64, AFD_PREC_MIXED = 0, 1
 AFD_C2P_PLAIN, AFD_C2P_LEARNED, AFD_C2P_PREVIOUS = 0, 1, 2
 AFD_BC_PERIODIC, AFD_BC_OUTFLOW = 0, 1
 AFD_OK, AFD_ERR_INVALID, AFD_ERR_NOMEM, AFD_ERR_HIP, AFD_ERR_STATE = 0, -1, -2, -3, -4
-AFD_FACE_PATH_DEFAULT = 2  # the face-kernel variant afd_params.face_path = 0 selects
+AFD_FACE_PATH_DEFAULT = 3  # the face-kernel variant afd_params.face_path = 0 selects
 AFD_NOALLOC_SCRATCH_BYTES = 16 << 20  # device staging of afd_c2p_batch_noalloc, per device
 
 

@@ -615,8 +615,12 @@ void k_face_rmhd_weno_lanes_div(FaceArgs<float> a) {
 #endif
 constexpr int MARCH_CELLS = AFD_MARCH_CELLS;
 
+#ifndef AFD_MARCH_WAVES
+#define AFD_MARCH_WAVES AFD_FACE_WAVES_F32
+#endif
+
 template <int RIEM, int DIR>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FaceOcc<float>::waves)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AFD_MARCH_WAVES)))
 void k_face_rmhd_weno_march(FaceArgs<float> a) {
     static_assert(DIR == 0 || DIR == 1, "pencils along x or y");
     constexpr int O = DIR == 0 ? 1 : 0;  // the transverse axis other than z
@@ -640,6 +644,16 @@ void k_face_rmhd_weno_march(FaceArgs<float> a) {
         const int buf = rd & 1, eidx = 4 * rd + w;
         const int e = m0 - 1 + min(eidx, count - 1);  // waves past the run repeat its last cell
         const unsigned c = (unsigned)(DIR == 0 ? cidx(g, e, o, k) : cidx(g, o, e, k));
+        // the face e is the right face of cell e - 1 (eidx >= 2) and the left face of cell e (eidx <= count - 2)
+        const bool right_of = eidx >= 2 && eidx < count, left_of = eidx >= 1 && eidx <= count - 2;
+        const bool upd = lane_ok && right_of;
+        const unsigned cl = c - s;
+        // x: cell e - 1's divergence so far, loaded before the face arithmetic (1.42 -> 1.21 ms a
+        // launch at 256^3); loaded at the end as in y, the loads' latency showed at every round.
+        // y keeps the late load (1.05 ms; 1.42 ms loaded early)
+        float dold[9];
+#pragma unroll
+        for (int v = 0; v < 9; ++v) dold[v] = (DIR == 0 && upd) ? a.D[v * n + cl] : 0.0f;
         State9 L, R;
         float lo[9], hi[9];
         weno5_cell_edges9(a.P, n, c, s, L, R, lo, hi);
@@ -655,8 +669,6 @@ void k_face_rmhd_weno_march(FaceArgs<float> a) {
         }
         RmhdFace<DIR, float> fc;
         rmhd_face_states<DIR, float>(a, L, R, fc);
-        // the face e is the right face of cell e - 1 (eidx >= 2) and the left face of cell e (eidx <= count - 2)
-        const bool right_of = eidx >= 2 && eidx < count, left_of = eidx >= 1 && eidx <= count - 2;
         float Fown[9], Fex[9];
         if constexpr (DIR == 0) {
             rmhd_face_pair<RIEM, 0>(a, fc, e, o, k, (long)c, right_of, left_of, Fown, Fex);
@@ -668,12 +680,11 @@ void k_face_rmhd_weno_march(FaceArgs<float> a) {
 #pragma unroll
         for (int v = 0; v < 9; ++v) flx[buf][w][v][lane] = Fex[v];
         __syncthreads();
-        if (lane_ok && right_of) {  // cell e - 1: right face e (Fown), left face e - 1 (the previous wave's)
-            const unsigned cl = c - s;
+        if (upd) {  // cell e - 1: right face e (Fown), left face e - 1 (the previous wave's)
 #pragma unroll
             for (int v = 0; v < 9; ++v) {
                 const float fl = w > 0 ? flx[buf][w - 1][v][lane] : flx[buf ^ 1][3][v][lane];
-                a.D[v * n + cl] += (Fown[v] - fl) * a.idn;
+                a.D[v * n + cl] = (DIR == 0 ? dold[v] : a.D[v * n + cl]) + (Fown[v] - fl) * a.idn;
             }
         }
     }
@@ -955,8 +966,9 @@ AFD_D double prim_Z(double gfac, double rho, double vx, double vy, double vz, do
 // Pin).  Either start must pass the acceptance test and is backed by the plain solve: inline, or (queue set) by queueing
 // the rejected cell for k_c2p_queue.  Reached by every lane of the wave (the network runs
 // wave-wide); `live` marks lanes with a cell.  Returns whether q holds the cell's primitives.
-template <int NB, bool LEARNED, typename F>
-AFD_D bool cell_c2p(const UpdArgs<F> &a, const double *U, bool live, long c, double *q, C2PStats &st) {
+template <int NB, bool LEARNED, typename F, bool LDSW = false>
+AFD_D bool cell_c2p(const UpdArgs<F> &a, const double *U, bool live, long c, double *q, C2PStats &st,
+                    const MlpLds *lw) {
     // The mixed RMHD con2prim uses the fast FP64 rcp/rsq: its hydro stage only seeds a Newton solve
     // accepted by its residual.  The SRHD con2prim's output is the fixed point's own branch (converged
     // or the energy fallback), which ulp-level changes can flip, so it stays IEEE.
@@ -967,7 +979,10 @@ AFD_D bool cell_c2p(const UpdArgs<F> &a, const double *U, bool live, long c, dou
         if constexpr (LEARNED) {
             float x[5], y[2];
             bool fok = mlp_features(U, x) && live;
-            mlp_infer_wave(mlp_view(a.mlp), x, y);
+            if constexpr (LDSW)  // the workgroup's LDS copy of the weights
+                mlp_infer_wave(MlpShared{lw}, x, y);
+            else
+                mlp_infer_wave(MlpGlobal{mlp_view(a.mlp)}, x, y);
             if (fok) done = rmhd_c2p_from_guess<FAST>(cp, U, mlp_guess_Z(U[0], a.gfac, y), q, st);
         } else {
             if (a.guess == 2 && live) {
@@ -988,20 +1003,12 @@ AFD_D bool cell_c2p(const UpdArgs<F> &a, const double *U, bool live, long c, dou
     return live;
 }
 
-// Stage update of one cell: flux divergence + sources, SSPRK combination into Uc (FP64), the
-// passives on primitives, and the con2prim of the new state into Pout.  LEARNED selects the
-// network's Newton start; a separate instantiation keeps its registers out of the plain kernel.
-// The learned instantiation is held to 4 waves per SIMD (<= 128 VGPRs; it takes 121, no
-// spills, with the network's two tiles evaluated one after the other): on the 256^3 RMHD
-// WENO5 + HLLD step it ran 16 % slower at the compiler's own 2 waves, and 4 % slower at 3 waves
-// with the tiles interleaved; 5 waves spills and is 25 % slower.  The plain one is held to 5
-// (<= 96 VGPRs, no spills; 100 registers and 4 waves otherwise): 6 % faster in FP32, 2.4 % in
-// FP64 on that step (6 waves spills and gains nothing more).
 // The stage update of cell c (a lane with live set) from its right-hand side: the SSPRK
 // combination into Uc (FP64), the passives on primitives and the con2prim of the new state into
-// Pout.  Reached by every lane of the wave (the learned start runs wave-wide).
-template <int NB, typename F, bool LEARNED>
-AFD_D void cell_update(const UpdArgs<F> &a, long c, bool live, const double *rhs) {
+// Pout.  Reached by every lane of the wave (the learned start runs wave-wide).  LDSW: the
+// network's forward weights come from the workgroup's LDS copy lw.
+template <int NB, typename F, bool LEARNED, bool LDSW = false>
+AFD_D void cell_update(const UpdArgs<F> &a, long c, bool live, const double *rhs, const MlpLds *lw = nullptr) {
     const Grid &g = a.g;
     C2PStats st = {0, 0, 0};
     double U[NB], q[NB];
@@ -1029,7 +1036,7 @@ AFD_D void cell_update(const UpdArgs<F> &a, long c, bool live, const double *rhs
             if (!last) a.Uc[v * g.ncell + c] = U[v];  // the last stage's U is read only by k_c2p_queue
         }
     }
-    const bool have = cell_c2p<NB, LEARNED>(a, U, live, c, q, st);
+    const bool have = cell_c2p<NB, LEARNED, F, LDSW>(a, U, live, c, q, st, lw);
     if (live) {
         if (have) {
 #pragma unroll
@@ -1065,7 +1072,7 @@ AFD_D void cell_update(const UpdArgs<F> &a, long c, bool live, const double *rhs
 // (<= 96 VGPRs, no spills; 100 registers and 4 waves otherwise): 6 % faster in FP32, 2.4 % in
 // FP64 on that step (6 waves spills and gains nothing more).
 #ifndef AFD_UPDATE_LEARNED_WAVES
-#define AFD_UPDATE_LEARNED_WAVES 4
+#define AFD_UPDATE_LEARNED_WAVES 5
 #endif
 template <int NB, typename F, bool LEARNED>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LEARNED ? AFD_UPDATE_LEARNED_WAVES : 5))) void k_update(
@@ -1073,9 +1080,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LEARNED ?
     const Grid &g = a.g;
     const long c = (long)blockIdx.x * BLOCK + threadIdx.x;
     const bool live = c < g.ncell;
-    double rhs[NB];
-    if (live) cell_rhs<NB, F>(a, c, in_range(g, c), rhs);
-    cell_update<NB, F, LEARNED>(a, c, live, rhs);
+    if constexpr (LEARNED) {
+        __shared__ MlpLds mw;  // the network's forward weights, read by every wave of the workgroup
+        mlp_stage_lds(a.mlp, mw, threadIdx.x, BLOCK);
+        double rhs[NB];
+        if (live) cell_rhs<NB, F>(a, c, in_range(g, c), rhs);
+        cell_update<NB, F, true, true>(a, c, live, rhs, &mw);
+    } else {
+        double rhs[NB];
+        if (live) cell_rhs<NB, F>(a, c, in_range(g, c), rhs);
+        cell_update<NB, F, false>(a, c, live, rhs);
+    }
 }
 
 // y pencils fused with the stage update (mixed RMHD WENO5 on the flux-divergence path): the pencil

@@ -113,7 +113,7 @@ __global__ __launch_bounds__(BLOCK) void k_mlp_forward(const void *w, const floa
     float x[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, y[2];
     if (sidx < n)
         for (int f = 0; f < 5; ++f) x[f] = X[5 * sidx + f];
-    mlp_infer_wave(mlp_view(w), x, y);  // every lane of the wave participates
+    mlp_infer_wave(MlpGlobal{mlp_view(w)}, x, y);  // every lane of the wave participates
     if (sidx < n) {
         Y[2 * sidx] = y[0];
         Y[2 * sidx + 1] = y[1];
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(64) void k_mlp_train_grad(const void *wp, const flo
             const float *yt = t ? ys : y;
             const bool tlive = base + 32 * t + col < n;
             MlpTile T;
-            mlp_tile_forward(w, lane, input_operand(xt, upper), T);
+            mlp_tile_forward(MlpGlobal{w}, lane, input_operand(xt, upper), T);
             float d0 = 0.f, d1 = 0.f;
             if (!upper && tlive) {
                 float e0 = T.o[0] - yt[0], e1 = T.o[1] - yt[1];
@@ -307,7 +307,7 @@ __global__ __launch_bounds__(BLOCK) void k_c2p_batch_learned(BatchArgs a, const 
     RmhdC2PParams cp = {a.gamma, a.gfac, a.p_max, a.v_max};
     float x[5], y[2];
     bool fok = mlp_features(U, x) && live;
-    mlp_infer_wave(mlp_view(mlp), x, y);
+    mlp_infer_wave(MlpGlobal{mlp_view(mlp)}, x, y);
     bool done = fok && rmhd_c2p_from_guess(cp, U, mlp_guess_Z(U[0], a.gfac, y), q, st);
     if (live && !done) rmhd_c2p(cp, U, q, st);
     if (live) {

@@ -51,6 +51,45 @@ AFD_D MlpDevW mlp_view(const void *w) {
     return v;
 }
 
+// Sources of the forward pass's weight operands: device memory (MlpGlobal) or a workgroup's LDS
+// copy (MlpShared).  blk(b, lane) is the A operand of block b (MlpBlock numbering) at a lane.
+struct MlpGlobal {
+    MlpDevW w;
+    AFD_D bf16x8 blk(int b, int lane) const { return w.blk[b * 64 + lane]; }
+    AFD_D float b3(int i) const { return w.bias[128 + i]; }
+};
+
+// The blocks the forward pass reads (W1, W2, W3, B2: 16 KB) and b3, staged in LDS by the
+// workgroups of the learned stage update.  Read from device memory, every wave's inference
+// fetched them again (32 KB a wave), as much traffic through the caches as the update's state.
+constexpr int MLP_FWD_BLK = 16;
+struct MlpLds {
+    bf16x8 blk[MLP_FWD_BLK][64];
+    float b3[2];
+};
+// LDS slot of a forward block: W1, W2, W3 (blocks 0..13) in place, B2 (24, 25) at 14, 15
+AFD_D int mlp_lds_slot(int b) { return b >= BLK_B2 ? b - BLK_B2 + 14 : b; }
+
+struct MlpShared {
+    const MlpLds *s;
+    AFD_D bf16x8 blk(int b, int lane) const { return s->blk[mlp_lds_slot(b)][lane]; }
+    AFD_D float b3(int i) const { return s->b3[i]; }
+};
+
+// Copies the forward blocks and b3 of the packed weights w to s.  Called by every thread of the
+// workgroup (tid in [0, nthreads)); ends with a barrier.
+AFD_D void mlp_stage_lds(const void *w, MlpLds &s, int tid, int nthreads) {
+    const bf16x8 *g = (const bf16x8 *)w;
+    const float *bias = (const float *)((const char *)w + (size_t)MLP_NBLK * 64 * 16);
+    for (int i = tid; i < MLP_FWD_BLK * 64; i += nthreads) {
+        const int slot = i >> 6, l = i & 63;
+        const int b = slot < 14 ? slot : BLK_B2 + (slot - 14);
+        s.blk[slot][l] = g[b * 64 + l];
+    }
+    if (tid < 2) s.b3[tid] = bias[128 + tid];
+    __syncthreads();
+}
+
 AFD_D floatx16 mfma_bf16(bf16x8 a, bf16x8 b, floatx16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
@@ -116,30 +155,30 @@ AFD_D bf16x8 ones_operand(bool upper) {
     return upper ? pack8(0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f) : pack8(1.f, 1.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f);
 }
 
-AFD_D void mlp_tile_forward(const MlpDevW &w, int lane, bf16x8 xb, MlpTile &t) {
+template <class WS> AFD_D void mlp_tile_forward(const WS &w, int lane, bf16x8 xb, MlpTile &t) {
     floatx16 h[2];
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt) {
-        t.z1[mt] = mfma_bf16(w.blk[(BLK_W1 + mt) * 64 + lane], xb, zero16());
+        t.z1[mt] = mfma_bf16(w.blk(BLK_W1 + mt, lane), xb, zero16());
 #pragma unroll
         for (int i = 0; i < 16; ++i) h[mt][i] = act(t.z1[mt][i]);
     }
     d_to_b(h, lane, t.h1b);
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt) {
-        floatx16 acc = mfma_bf16(w.blk[(BLK_B2 + mt) * 64 + lane], ones_operand(lane >= 32), zero16());
+        floatx16 acc = mfma_bf16(w.blk(BLK_B2 + mt, lane), ones_operand(lane >= 32), zero16());
 #pragma unroll
-        for (int kb = 0; kb < 4; ++kb) acc = mfma_bf16(w.blk[(BLK_W2 + 4 * mt + kb) * 64 + lane], t.h1b[kb], acc);
+        for (int kb = 0; kb < 4; ++kb) acc = mfma_bf16(w.blk(BLK_W2 + 4 * mt + kb, lane), t.h1b[kb], acc);
 #pragma unroll
         for (int i = 0; i < 16; ++i) h[mt][i] = act(acc[i]);
         t.z2[mt] = acc;
     }
     d_to_b(h, lane, t.h2b);
     floatx16 o = zero16();
-    o[0] = w.bias[128];
-    o[1] = w.bias[129];
+    o[0] = w.b3(0);
+    o[1] = w.b3(1);
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb) o = mfma_bf16(w.blk[(BLK_W3 + kb) * 64 + lane], t.h2b[kb], o);
+    for (int kb = 0; kb < 4; ++kb) o = mfma_bf16(w.blk(BLK_W3 + kb, lane), t.h2b[kb], o);
     t.o = o;
 }
 
@@ -150,7 +189,7 @@ AFD_D bf16x8 input_operand(const float x[5], bool upper) {
 
 // Whole-wave inference: lane l supplies the features of its sample, gets its two outputs.
 // Must be reached by all 64 lanes of the wave.
-AFD_D void mlp_infer_wave(const MlpDevW &w, const float x[5], float y[2]) {
+template <class WS> AFD_D void mlp_infer_wave(const WS &w, const float x[5], float y[2]) {
     const int lane = threadIdx.x & 63;
     const bool upper = lane >= 32;
     float xs[5];
@@ -161,9 +200,11 @@ AFD_D void mlp_infer_wave(const MlpDevW &w, const float x[5], float y[2]) {
     // one tile after the other: interleaved by the scheduler, the two tiles' activations took the
     // stage update to 163 registers (3 waves per SIMD); one at a time it needs 121 (4 waves)
     __builtin_amdgcn_sched_barrier(0);
-    MlpDevW w1 = w;  // the second tile loads the weights again rather than keep the first tile's live
-    asm volatile("" : "+s"(w1.blk));
-    mlp_tile_forward(w1, lane, input_operand(xs, upper), t1);
+    // the second tile loads the weights again rather than keep the first tile's live: an opaque
+    // copy of the lane index keeps the compiler from reusing the first tile's loads
+    int lane1 = lane;
+    asm volatile("" : "+v"(lane1));
+    mlp_tile_forward(w, lane1, input_operand(xs, upper), t1);
     float a0 = __shfl_xor(t1.o[0], 32), a1 = __shfl_xor(t1.o[1], 32);
     y[0] = upper ? a0 : t0.o[0];
     y[1] = upper ? a1 : t0.o[1];

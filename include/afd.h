@@ -86,7 +86,7 @@ typedef struct afd_params {
 } afd_params;
 
 /* The variant face_path 0 selects (the fastest measured on the 256^3 RMHD WENO5 step). */
-#define AFD_FACE_PATH_DEFAULT 2
+#define AFD_FACE_PATH_DEFAULT 3
 
 /* Nozzle inflow on the -x ghost planes (core/nozzle.py:27-92).  The library's host code
  * evaluates nozzle_weight (core/nozzle.py:5-25) per (j, k) and uploads the profile. */
