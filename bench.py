@@ -203,6 +203,11 @@ def accuracy(sp, sm, pr0, checkpoints, mlp=None, flips=False):
                 e["step"] = n
                 e["nonfinite"] = {"plain": int(np.count_nonzero(~np.isfinite(b))),
                                   "mixed": int(np.count_nonzero(~np.isfinite(a)))}
+                # the largest density of each run: where the scheme's own solution has left the
+                # range an FP32 flux can carry (|F| < 3.4e38), the mixed path's fluxes overflow
+                ng = sp["NG"]
+                e["rho_max"] = {"plain": float(np.nanmax(b[0][(slice(ng, -ng),) * 3])),
+                                "mixed": float(np.nanmax(np.where(np.isfinite(a[0]), a[0], np.nan)[(slice(ng, -ng),) * 3]))}
                 del a
                 q = dq.get_prim()
                 e["fp64_ulp_spread"] = {k: {"L1_rel": v["L1_rel"], "max_abs": v["max_abs"]}
@@ -562,15 +567,22 @@ def c2p_report(st):
 
 
 def copy_probe(gib):
-    """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    """Plain device kernels of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters: a copy
+    (reads and writes n x 8 B), a sum (reads n x 8 B) and a fill (writes n x 8 B), three launches each."""
     import torch
     n = int(gib * (1 << 30)) // 8
     a = torch.ones(n, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
     for _ in range(3):
         b.copy_(a)
+    s = None
+    for _ in range(3):
+        s = a.sum()
+    for _ in range(3):
+        b.fill_(2.0)
     torch.cuda.synchronize()
-    log(f"[bench] copy probe: {n * 8} bytes read + {n * 8} bytes written per copy")
+    log(f"[bench] copy probe: {n * 8} bytes read + {n * 8} bytes written per copy; sum of {n * 8} bytes "
+        f"({float(s):.0f}); fill of {n * 8} bytes")
 
 
 def main():
