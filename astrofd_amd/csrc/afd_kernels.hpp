@@ -1082,7 +1082,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LEARNED ?
     const bool live = c < g.ncell;
     if constexpr (LEARNED) {
         __shared__ MlpLds mw;  // the network's forward weights, read by every wave of the workgroup
-        mlp_stage_lds(a.mlp, mw, threadIdx.x, BLOCK);
+        mlp_stage_lds<BLOCK>(a.mlp, mw, threadIdx.x);
         double rhs[NB];
         if (live) cell_rhs<NB, F>(a, c, in_range(g, c), rhs);
         cell_update<NB, F, true, true>(a, c, live, rhs, &mw);

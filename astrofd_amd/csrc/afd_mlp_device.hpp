@@ -76,17 +76,28 @@ struct MlpShared {
     AFD_D float b3(int i) const { return s->b3[i]; }
 };
 
-// Copies the forward blocks and b3 of the packed weights w to s.  Called by every thread of the
-// workgroup (tid in [0, nthreads)); ends with a barrier.
-AFD_D void mlp_stage_lds(const void *w, MlpLds &s, int tid, int nthreads) {
+// Copies the forward blocks and b3 of the packed weights w to s.  Called by every thread of a
+// workgroup of NT threads (tid in [0, NT)); ends with a barrier.  The loads are all issued before
+// the first store, so that a workgroup waits for one memory round trip, not one per copy.
+template <int NT> AFD_D void mlp_stage_lds(const void *w, MlpLds &s, int tid) {
+    static_assert((MLP_FWD_BLK * 64) % NT == 0, "whole copies per thread");
+    constexpr int PER = MLP_FWD_BLK * 64 / NT;
     const bf16x8 *g = (const bf16x8 *)w;
     const float *bias = (const float *)((const char *)w + (size_t)MLP_NBLK * 64 * 16);
-    for (int i = tid; i < MLP_FWD_BLK * 64; i += nthreads) {
-        const int slot = i >> 6, l = i & 63;
+    bf16x8 v[PER];
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int i = tid + r * NT, slot = i >> 6, l = i & 63;
         const int b = slot < 14 ? slot : BLK_B2 + (slot - 14);
-        s.blk[slot][l] = g[b * 64 + l];
+        v[r] = g[b * 64 + l];
     }
-    if (tid < 2) s.b3[tid] = bias[128 + tid];
+    const float b3 = tid < 2 ? bias[128 + tid] : 0.0f;
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int i = tid + r * NT;
+        s.blk[i >> 6][i & 63] = v[r];
+    }
+    if (tid < 2) s.b3[tid] = b3;
     __syncthreads();
 }
 

@@ -727,6 +727,9 @@ def main():
             # show how the reference scheme's own divergence (non-finite counts) takes over
             res["errors_vs_fp64_plain"] = ev[0]
             res["errors_evolution"] = ev
+            res["accuracy_scope"] = ("the metric's L1(rho) <= 1e-6 is for one step from the seeded state, the timed "
+                                     "unit; errors_evolution and hlle_50 give later steps beside the FP64 run's own "
+                                     "response to 1-ulp input noise (fp64_ulp_spread)")
             res["fp64_ulp_spread"] = dict(ev[0]["fp64_ulp_spread"], step=1)
         if a.evolve_steps > 0 and not a.evolve:
             zu_e = s["NX"] * s["NY"] * s["NZ"] * a.evolve_steps
