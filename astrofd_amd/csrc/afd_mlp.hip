@@ -185,14 +185,14 @@ __global__ __launch_bounds__(64) void k_mlp_train_grad(const void *wp, const flo
             floatx16 dz2[2], dz1[2];
             for (int mt = 0; mt < 2; ++mt) {
                 dz2[mt] = mfma_bf16(w.blk[(BLK_W3T + mt) * 64 + lane], dob, zero16());
-                for (int i = 0; i < 16; ++i) dz2[mt][i] *= dact(T.z2[mt][i]);
+                for (int i = 0; i < 16; ++i) dz2[mt][i] *= dsilu(T.z2[mt][i]);
             }
             bf16x8 dz2b[4];
             d_to_b(dz2, lane, dz2b);
             for (int mt = 0; mt < 2; ++mt) {
                 floatx16 acc = zero16();
                 for (int kb = 0; kb < 4; ++kb) acc = mfma_bf16(w.blk[(BLK_W2T + 4 * mt + kb) * 64 + lane], dz2b[kb], acc);
-                for (int i = 0; i < 16; ++i) acc[i] *= dact(T.z1[mt][i]);
+                for (int i = 0; i < 16; ++i) acc[i] *= dsilu(T.z1[mt][i]);
                 dz1[mt] = acc;
             }
             for (int mt = 0; mt < 2; ++mt)
@@ -200,8 +200,8 @@ __global__ __launch_bounds__(64) void k_mlp_train_grad(const void *wp, const flo
                     int row = 32 * mt + drow(i, lane);
                     gb2[mt][i] += dz2[mt][i];
                     gb1[mt][i] += dz1[mt][i];
-                    sH1[row * LDS_S + col] = (__bf16)act(T.z1[mt][i]);
-                    sH2[row * LDS_S + col] = (__bf16)act(T.z2[mt][i]);
+                    sH1[row * LDS_S + col] = (__bf16)silu(T.z1[mt][i]);
+                    sH2[row * LDS_S + col] = (__bf16)silu(T.z2[mt][i]);
                     sDZ2[row * LDS_S + col] = (__bf16)dz2[mt][i];
                     sDZ1[row * LDS_S + col] = (__bf16)dz1[mt][i];
                 }

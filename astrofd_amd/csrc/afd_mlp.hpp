@@ -1,6 +1,6 @@
 // afd_mlp.hpp -- the learned con2prim guess: storage, training and inference launchers.
 //
-// MLP 5 -> 64 -> 64 -> 2 with ReLU.  Packed FP32 master layout (4674 values):
+// MLP 5 -> 64 -> 64 -> 2 with SiLU.  Packed FP32 master layout (4674 values):
 //   W1[64][5], b1[64], W2[64][64], b2[64], W3[2][64], b3[2]
 // The device keeps the FP32 master, Adam moments, a gradient buffer and the BF16 copy of
 // the weight matrices laid out as MFMA A operands (afd_mlp_device.hpp) followed by the

@@ -1,4 +1,4 @@
-// afd_mlp_device.hpp -- wave-level BF16 MFMA multilayer perceptron (5 -> 64 -> 64 -> 2, ReLU).
+// afd_mlp_device.hpp -- wave-level BF16 MFMA multilayer perceptron (5 -> 64 -> 64 -> 2, SiLU).
 //
 // One wavefront evaluates 64 samples as two tiles of 32 with v_mfma_f32_32x32x16_bf16.
 // Activations are kept transposed (features x samples) so that samples run along the
@@ -114,12 +114,12 @@ AFD_D floatx16 zero16() {
 
 AFD_D int drow(int i, int lane) { return 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3); }
 
-// The activation is ReLU, one v_max_f32 per value.  The network's cost was its 128 activations
-// per sample: as SiLU (an exp and a reciprocal, each about three FMAs' issue time on gfx950)
-// they took three quarters of an inference, while the guess it feeds only seeds Newton.
-AFD_D float act(float z) { return fmaxf(z, 0.0f); }
-// d/dz relu(z): 1 for z > 0, else 0 (torch's convention at z = 0)
-AFD_D float dact(float z) { return z > 0.0f ? 1.0f : 0.0f; }
+AFD_D float silu(float x) { return x * __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
+// d/dx silu(x) = s (1 + x (1 - s)), s = sigmoid(x)
+AFD_D float dsilu(float x) {
+    float s = __builtin_amdgcn_rcpf(1.0f + __expf(-x));
+    return s * (1.0f + x * (1.0f - s));
+}
 
 AFD_D bf16x8 pack8(float a0, float a1, float a2, float a3, float a4, float a5, float a6, float a7) {
     bf16x8 r;
@@ -172,7 +172,7 @@ template <class WS> AFD_D void mlp_tile_forward(const WS &w, int lane, bf16x8 xb
     for (int mt = 0; mt < 2; ++mt) {
         t.z1[mt] = mfma_bf16(w.blk(BLK_W1 + mt, lane), xb, zero16());
 #pragma unroll
-        for (int i = 0; i < 16; ++i) h[mt][i] = act(t.z1[mt][i]);
+        for (int i = 0; i < 16; ++i) h[mt][i] = silu(t.z1[mt][i]);
     }
     d_to_b(h, lane, t.h1b);
 #pragma unroll
@@ -181,7 +181,7 @@ template <class WS> AFD_D void mlp_tile_forward(const WS &w, int lane, bf16x8 xb
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb) acc = mfma_bf16(w.blk(BLK_W2 + 4 * mt + kb, lane), t.h1b[kb], acc);
 #pragma unroll
-        for (int i = 0; i < 16; ++i) h[mt][i] = act(acc[i]);
+        for (int i = 0; i < 16; ++i) h[mt][i] = silu(acc[i]);
         t.z2[mt] = acc;
     }
     d_to_b(h, lane, t.h2b);

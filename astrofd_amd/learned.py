@@ -1,6 +1,6 @@
 """Learned con2prim guess: sampling, GPU training and evaluation (no counterpart in the reference).
 
-The network (5 -> 64 -> 64 -> 2, ReLU, BF16 MFMA kernels with FP32 accumulation in
+The network (5 -> 64 -> 64 -> 2, SiLU, BF16 MFMA kernels with FP32 accumulation in
 libafd) maps the scaled conserved state of a cell to (log10(p/D)/4, log10 W).  The stage
 update turns that into the Newton start Z0 = D W + Gamma/(Gamma-1) p W^2 and runs the
 reference's FD-Newton iteration (core/rmhd_core.py:273-290) from it; the acceptance test passes the

@@ -466,7 +466,7 @@ BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
 
 
 def network_rate(mlp, n=1 << 24, launches=3):
-    """The learned guess's MLP (5 -> 64 -> 64 -> 2, ReLU, BF16 weights, FP32 accumulation) alone over
+    """The learned guess's MLP (5 -> 64 -> 64 -> 2, SiLU, BF16 weights, FP32 accumulation) alone over
     n samples: kernel time from HIP events on the library's stream, achieved FLOP rate against the
     dense BF16 MFMA peak.  The inputs are uniform in [-1, 1] (the rate does not depend on them)."""
     from astrofd_amd.domain import Domain

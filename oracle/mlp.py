@@ -4,7 +4,7 @@ TEST INFRASTRUCTURE ONLY: the tests, the smoke check and the CPU leg of the
 benchmark use this module as the checker of the GPU kernels; nothing in the
 product imports it.
 
-The network (no counterpart in the reference): 5 -> 64 -> 64 -> 2 with ReLU,
+The network (no counterpart in the reference): 5 -> 64 -> 64 -> 2 with SiLU,
 parameters in one FP32 vector  W1[64][5], b1[64], W2[64][64], b2[64],
 W3[2][64], b3[2]  (4674 values).  Loss = mean over samples of the summed
 squared error of the two outputs.
@@ -13,7 +13,8 @@ squared error of the two outputs.
 (weights, inputs, hidden activations, back-propagated deltas; accumulation is
 FP32, here FP64; the biases b1 and b2 enter as the pair hi + lo of BF16 values,
 b1 and b2 of 16 significant bits, and b3 in FP32), so that the GPU result
-differs from the oracle only by FP32 accumulation order.
+differs from the oracle only by FP32 accumulation order and the hardware
+exp/reciprocal.
 `emulate=False` is the plain FP64 network, pinned in the tests by central
 finite differences of the loss and by torch autograd on the CPU.
 
@@ -69,13 +70,17 @@ def bias_split(b):
     return hi + bf16(b - hi)
 
 
-def relu(z):
-    return np.maximum(z, 0.0)
+def sigmoid(z):
+    return 0.5 * (1.0 + np.tanh(0.5 * z))
 
 
-def drelu(z):
-    """d/dz relu: 1 for z > 0, else 0 (torch's convention at z = 0)."""
-    return (np.asarray(z) > 0.0).astype(np.float64)
+def silu(z):
+    return z * sigmoid(z)
+
+
+def dsilu(z):
+    s = sigmoid(z)
+    return s * (1.0 + z * (1.0 - s))
 
 
 def forward(w, X, emulate=True):
@@ -85,9 +90,9 @@ def forward(w, X, emulate=True):
     W1, b1, W2, b2, W3, b3 = unpack(w)
     xb = r(X)                                  # n x 5
     z1 = xb @ r(W1).T + rb(b1)                 # n x 64
-    h1 = r(relu(z1))
+    h1 = r(silu(z1))
     z2 = h1 @ r(W2).T + rb(b2)
-    h2 = r(relu(z2))
+    h2 = r(silu(z2))
     o = h2 @ r(W3).T + b3
     return o, dict(xb=xb, z1=z1, h1=h1, z2=z2, h2=h2)
 
@@ -102,9 +107,9 @@ def loss_grad(w, X, Y, emulate=True):
     loss = float(np.sum(e * e) / n)
     d = 2.0 * e / n                           # dL/do
     db = r(d)
-    dz2 = (db @ r(W3)) * drelu(t["z2"])
+    dz2 = (db @ r(W3)) * dsilu(t["z2"])
     dz2b = r(dz2)
-    dz1 = (dz2b @ r(W2)) * drelu(t["z1"])
+    dz1 = (dz2b @ r(W2)) * dsilu(t["z1"])
     dz1b = r(dz1)
     g = pack(dz1b.T @ t["xb"], dz1.sum(0), dz2b.T @ t["h1"], dz2.sum(0), db.T @ t["h2"], d.sum(0))
     return loss, g

@@ -55,8 +55,8 @@ def test_forward_vs_oracle(ctx, n):
     far = np.mean(d > 1e-5)
     log(f"  max |gpu - oracle(bf16)| {d.max():.3e}  mean {d.mean():.3e}  fraction > 1e-5: {far:.2e}  "
         f"|gpu - fp64 net| {d64:.3e}")
-    # Outputs agree to FP32 accumulation error (~1e-7) except where the accumulation order
-    # moves the BF16 rounding of a hidden activation by one BF16 ulp
+    # Outputs agree to FP32 accumulation error (~1e-7) except where the accumulation order or
+    # the hardware exp/rcp moves the BF16 rounding of a hidden activation by one BF16 ulp
     # (2^-8 relative); through the next layers that moves an output by a few 1e-3 at most,
     # in a small fraction of samples.  A wrong operand layout or index breaks nearly all.
     assert d.max() <= 5e-3

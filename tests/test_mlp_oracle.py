@@ -32,8 +32,8 @@ def _torch_net(w):
 def _torch_loss(params, X, Y):
     W1, b1, W2, b2, W3, b3 = params
     x = torch.tensor(X, dtype=torch.float64)
-    h1 = torch.relu(x @ W1.T + b1)
-    h2 = torch.relu(h1 @ W2.T + b2)
+    h1 = torch.nn.functional.silu(x @ W1.T + b1)
+    h2 = torch.nn.functional.silu(h1 @ W2.T + b2)
     o = h2 @ W3.T + b3
     return ((o - torch.tensor(Y)) ** 2).sum() / X.shape[0], o
 
