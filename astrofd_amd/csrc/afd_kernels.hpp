@@ -564,8 +564,12 @@ AFD_D void rmhd_face_pair(const FaceArgs<float> &a, const RmhdFace<DIR, float> &
 // 1..62 write D for their cells (k = r0 .. r1 - 1 of a line) and the z-back speeds of their faces.
 // Every lane evaluates a face (lanes without one use in-range duplicates and write nothing), so
 // that every lane takes part in the shuffles.
+#ifndef AFD_LANES_DIV_WAVES
+#define AFD_LANES_DIV_WAVES AFD_FACE_WAVES_F32
+#endif
+
 template <int RIEM>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(FaceOcc<float>::waves)))
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(AFD_LANES_DIV_WAVES)))
 void k_face_rmhd_weno_lanes_div(FaceArgs<float> a) {
     const Grid &g = a.g;
     const int ncz = g.r1[2] - g.r0[2] + 2, ny = g.r1[1] - g.r0[1];
