@@ -77,10 +77,21 @@ template <typename T> AFD_D T minmod(T a, T b) {
 template <typename T> AFD_D T limiter(int lid, T dqL, T dqR) {
     if (lid == 1) return minmod(dqL, dqR);
     if (lid == 2) {
-        if (dqL * dqR <= T(0)) return T(0);
-        T denom = dqL + dqR;
-        if (denom == T(0)) return T(0);
-        return fdiv(T(2) * dqL * dqR, denom);
+        if constexpr (sizeof(T) == 4) {
+            // FP32 van Leer as 2 dqL (dqR / (dqL + dqR)): the quotient lies in (0, 1) when the
+            // slopes share a sign, so the slope stays finite where the product of two slopes
+            // above ~1.8e19 overflows (2 dqL dqR / (dqL + dqR) then gives inf or NaN).  The
+            // product still decides the sign test: where it underflows (both slopes below
+            // ~1e-22) the slope is 0, as before, and where it passes, |dqL + dqR| >= 3.7e-23, so
+            // its reciprocal is finite (a denormal sum's is not)
+            if (!(dqL * dqR > T(0))) return T(0);
+            return T(2) * dqL * fdiv(dqR, dqL + dqR);
+        } else {
+            if (dqL * dqR <= T(0)) return T(0);
+            T denom = dqL + dqR;
+            if (denom == T(0)) return T(0);
+            return fdiv(T(2) * dqL * dqR, denom);
+        }
     }
     T mm = minmod(dqL, dqR);
     return minmod(T(0.5) * (dqL + dqR), T(2) * mm);

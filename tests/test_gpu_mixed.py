@@ -114,6 +114,47 @@ def test_mixed_weno_large_jumps(phys, riem, face_path):
                      finite_only=riem == "hlld_full")
 
 
+@pytest.mark.parametrize("phys,riem", [("hydro", "hllc"), ("rmhd", "hlle")])
+def test_mixed_vanleer_large_jumps(phys, riem):
+    """MUSCL with the van Leer limiter across density jumps of 1e20: the FP32 slopes' product
+    (above ~1.8e19 squared) overflows, and 2 a b / (a + b) gave inf / NaN slopes where FP64 is
+    finite; the FP32 limiter forms 2 a (b / (a + b)) instead.  The SRHD 256^3 jet's own state
+    reaches such contrasts (densities of 1e-12 next to 1e18) by step 90.  Bars as for the WENO5
+    jumps: rho's L1, and every conserved component within 1e-5 max(tau + D)."""
+    log(f"test_mixed_vanleer_large_jumps {phys} {riem}")
+    check_mixed_step(phys, "muscl", "vanleer", riem, 3, (22, 20, 18), vamp=0.2, jump=1e20)
+
+
+def test_mixed_vanleer_tiny_slopes():
+    """MUSCL van Leer in FP32 where both slopes are tiny and share a sign: velocity deviations of a
+    few 1e-39 (FP32 denormals), whose sum's reciprocal is not finite.  The slope must come out 0
+    (the product of the slopes underflows), not inf or NaN.  Bars as in check_mixed_step."""
+    log("test_mixed_vanleer_tiny_slopes")
+    ng = 3
+    pr = smooth_state((22 + 2 * ng, 20 + 2 * ng, 18 + 2 * ng), 97, "hydro", vamp=0.2)
+    ramp = np.arange(pr.shape[1], dtype=np.float64)[:, None, None]
+    pr[2] = 4e-39 * ramp * np.ones(pr.shape[1:])  # vy: same-sign deviations of 4e-39 along x
+    pr[3] = 3e-39 * np.arange(pr.shape[3], dtype=np.float64)[None, None, :] * np.ones(pr.shape[1:])
+    dx = 1.0 / 22
+    dt = 0.4 * dx * 0.5
+    d = make_domain("hydro", pr.shape, ng, dxyz=(dx, dx, dx), recon="muscl", limiter="vanleer", riemann="hllc",
+                    rk_order=3, precision="mixed")
+    d.set_prim(pr)
+    d.step(dt)
+    out = d.get_prim()
+    d.close()
+    P = O.make_params("hydro", pr.shape[1:], nv=5, ng=ng, p_max=2.0, v_max=0.9999, limiter="vanleer", recon="muscl",
+                      riemann="hllc", rk_order=3)
+    ref = O.step(P, pr, dx, dx, dx, dt)
+    core = (slice(ng, -ng),) * 3
+    assert np.all(np.isfinite(out))
+    for name, a, b in (("rho", out[0], ref[0]), ("p", out[4], ref[4]), ("W", _lorentz(out), _lorentz(ref))):
+        a, b = a[core], b[core]
+        l1 = np.abs(a - b).sum() / np.abs(b).sum()
+        log(f"  {name}: L1 rel {l1:.2e}")
+        assert l1 <= (1e-6 if name == "rho" else 1e-5), name
+
+
 def check_mixed_step(phys, recon, lim, riem, rk, interior, vamp=0.5, npass=0, face_path=0, jump=None,
                      finite_only=False):
     ng = 3
