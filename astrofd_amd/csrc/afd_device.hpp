@@ -291,6 +291,17 @@ AFD_D void hllc5(const T *UL, const T *UR, const T *FL, const T *FR, T sL, T sR,
     Ust[4] = U[4] * fac + fdiv(pStar * sM - p * v, den);
 #pragma unroll
     for (int m = 0; m < 5; ++m) out[m] = F[m] + s * (Ust[m] - U[m]);
+    if constexpr (sizeof(T) == 4) {
+        // FP32 only: at a face that opens a vacuum (a cold dense state, whose slower wave speed
+        // rounds onto its velocity, beside a near-vacuum state) the contact-speed denominator
+        // can cancel to 0, and the star pressure's (s - v)(sM - v) is then 0 x inf.  Where the
+        // star state is not finite the face takes the HLLE flux (FP64, whose denominator does not
+        // cancel, keeps the reference's HLLC)
+        bool ok = true;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) ok = ok && isfinite(out[m]);
+        if (!ok) hlle_vec<5>(UL, UR, FL, FR, sL, sR, out);
+    }
 }
 
 // ---------------------------------------------------------------- Lorentz factor (FP64)
@@ -479,7 +490,7 @@ template <typename T> AFD_D void rmhd_p2c_state(const Phys<T> &ph, const T *q, T
 // (:703-714) are never used and are not computed; only the star states the selected
 // region needs are converted (ULs/Us on the left of the contact, URs/Us on the right).
 template <int DIR, typename T>
-AFD_D void hlld_full(const Phys<T> &ph, const T *UL, const T *UR, const T *FL, const T *FR, T sL, T sR, T pL,
+AFD_D void hlld_full_regions(const Phys<T> &ph, const T *UL, const T *UR, const T *FL, const T *FR, T sL, T sR, T pL,
                      T pR, T vL, T vR, T *out) {
     if (sL >= T(0)) {
 #pragma unroll
@@ -580,6 +591,21 @@ AFD_D void hlld_full(const Phys<T> &ph, const T *UL, const T *UR, const T *FL, c
     T ss = onL ? sL_star : sR_star;
 #pragma unroll
     for (int m = 0; m < 9; ++m) out[m] = F[m] + s * (Uo[m] - U[m]) + ss * (Us[m] - Uo[m]);
+}
+
+// hll_d_flux_dir.  In FP32 a face whose star states are not finite (the contact-speed denominator
+// cancelling to 0 at a face that opens a vacuum, as in hllc5) takes the HLLE flux; FP64 is the
+// reference's solver unchanged.
+template <int DIR, typename T>
+AFD_D void hlld_full(const Phys<T> &ph, const T *UL, const T *UR, const T *FL, const T *FR, T sL, T sR, T pL,
+                     T pR, T vL, T vR, T *out) {
+    hlld_full_regions<DIR>(ph, UL, UR, FL, FR, sL, sR, pL, pR, vL, vR, out);
+    if constexpr (sizeof(T) == 4) {
+        bool ok = true;
+#pragma unroll
+        for (int m = 0; m < 9; ++m) ok = ok && isfinite(out[m]);
+        if (!ok) hlle_vec<9>(UL, UR, FL, FR, sL, sR, out);
+    }
 }
 
 // ---------------------------------------------------------------- con2prim (FP64)

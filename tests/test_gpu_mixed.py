@@ -155,6 +155,60 @@ def test_mixed_vanleer_tiny_slopes():
         assert l1 <= (1e-6 if name == "rho" else 1e-5), name
 
 
+# Seven cells along x of a state the SRHD 256^3 jet's mixed run reached (step 72): a cold dense
+# cell (rho 5.4e5, p 1e-12) moving left beside a near-vacuum cell moving right.  (rho, vx, vy, vz, p)
+VACUUM_ROW = np.array([
+    [1.43245e+04, 9.69516e-01, 9.27624e-02, -1.43041e-01, 1.00000e-12],
+    [5.40249e+05, -4.66499e-01, -5.39392e-01, 6.28299e-01, 1.00000e-12],
+    [1.00000e-12, 5.00162e-01, 6.58097e-01, -5.62624e-01, 1.00000e-12],
+    [2.14480e+03, -5.12765e-01, 7.96350e-01, 3.20468e-01, 2.00000e+00],
+    [2.57216e+04, -2.87034e-01, -7.42998e-01, -6.04455e-01, 1.50000e-12],
+    [3.15012e+04, -5.77672e-01, -1.61371e-02, 8.15987e-01, 1.00000e-12],
+    [1.00000e-12, 6.13623e-01, -1.24587e-01, -7.79580e-01, 1.00000e-12],
+])
+
+
+@pytest.mark.parametrize("phys,riem", [("hydro", "hllc"), ("rmhd", "hlld")])
+def test_mixed_hllc_vacuum_faces(phys, riem):
+    """FP32 HLLC at faces that open a vacuum: the cold dense cell's slower wave speed rounds onto
+    its velocity in FP32, the contact-speed denominator cancels to 0 and the star pressure became
+    0 x inf = NaN where FP64 is finite.  Such faces take the HLLE flux in FP32 (FP64 keeps HLLC).
+    The rows above, repeated through y and z in a smooth state, with MUSCL van Leer: every face of
+    their vacuum planes is such a face, so the rows' cells differ from the FP64 HLLC step by what
+    HLLE changes there (about 20 % of their density here; the jet has a few dozen such faces among
+    10^8).  The step must be finite and non-negative in rho and p, and the cells outside the rows'
+    reach (3 stages of MUSCL) within the usual bars."""
+    log(f"test_mixed_hllc_vacuum_faces {phys} {riem}")
+    ng = 2
+    shape = (22 + 2 * ng, 12 + 2 * ng, 12 + 2 * ng)
+    pr = smooth_state(shape, 98, phys, vamp=0.2)
+    for r, row in enumerate(VACUUM_ROW):
+        for v in range(5):
+            pr[v][ng + 6 + r] = row[v]
+    gamma = 4.0 / 3.0 if phys == "rmhd" else None
+    dx = 1.0 / 256
+    dt = 0.4 * dx
+    d = make_domain(phys, pr.shape, ng, dxyz=(dx, dx, dx), gamma=gamma, recon="muscl", limiter="vanleer",
+                    riemann=riem, rk_order=3, precision="mixed")
+    d.set_prim(pr)
+    d.step(dt)
+    out = d.get_prim()
+    d.close()
+    P = O.make_params(phys, pr.shape[1:], nv=pr.shape[0], ng=ng, gamma=gamma, p_max=2.0, v_max=0.9999,
+                      limiter="vanleer", recon="muscl", riemann=riem, rk_order=3)
+    ref = O.step(P, pr, dx, dx, dx, dt)
+    core = (slice(ng, -ng),) * 3
+    assert np.all(np.isfinite(out))
+    assert out[0][core].min() > 0.0 and out[4][core].min() > 0.0
+    # the smooth part of the field, x planes away from the vacuum rows' stencils (3 stages of MUSCL)
+    far = (slice(ng + 19, -ng),) + (slice(ng, -ng),) * 2
+    for v in range(5):
+        a, b = out[v][far], ref[v][far]
+        l1 = np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-300)
+        log(f"  var {v} away from the rows: L1 rel {l1:.2e}")
+        assert l1 <= 1e-5, v
+
+
 def check_mixed_step(phys, recon, lim, riem, rk, interior, vamp=0.5, npass=0, face_path=0, jump=None,
                      finite_only=False):
     ng = 3
