@@ -57,7 +57,13 @@ struct afd_ctx {
     unsigned noz_mask = 0;
     // reductions
     int red_blocks = 1024;
-    double *red_partial = nullptr, *red_out = nullptr;
+    double *red_out = nullptr;
+    // the CFL speed's maximum over the speed box, as double bits in AMAX_SLOTS slots, from an SRHD
+    // step's last stage or from k_max_speed_box, valid until something other than a step or a
+    // fill changes the state; then AMAX_SLOTS slots of the shell's maximum, zero between calls
+    unsigned long long *amax_dev = nullptr;
+    SpeedBox sbox;
+    bool speed_valid = false;
     unsigned long long *cnt_partial = nullptr, *cnt_out = nullptr;
     double *diag_partial = nullptr, *diag_out = nullptr;
     // instrumentation
@@ -243,6 +249,14 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
             return fail(AFD_ERR_INVALID, "grid too small for the reconstruction stencil");
         }
     }
+    bool box_empty = false;
+    for (int a = 0; a < 3; ++a) {
+        c->sbox.lo[a] = std::max(g.r0[a], p.ng);
+        c->sbox.hi[a] = std::min(g.r1[a], full[a] - p.ng);
+        box_empty |= c->sbox.hi[a] <= c->sbox.lo[a];
+    }
+    if (box_empty)  // every cell of [1, n-1)^3 then goes to k_max_speed_shell
+        for (int a = 0; a < 3; ++a) c->sbox.lo[a] = c->sbox.hi[a] = 1;
     c->carried = nb == 9 && p.recon_id != AFD_RECON_MUSCL;
     c->passive_flux = p.n_passive > 0 && (nb == 9 ? p.recon_id == AFD_RECON_MUSCL : p.recon_id != AFD_RECON_WENO);
     c->fsize = p.precision == AFD_PREC_MIXED ? 4 : 8;
@@ -284,8 +298,12 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
         for (int d = 0; d < 3; ++d) TRY(dalloc(c, (void **)&c->aux[d], N * 4 * 8));
         TRY(dalloc(c, (void **)&c->rhs_pass, N * p.n_passive * 8));
     }
-    TRY(dalloc(c, (void **)&c->red_partial, c->red_blocks * 8));
     TRY(dalloc(c, (void **)&c->red_out, 8));
+    TRY(dalloc(c, (void **)&c->amax_dev, AMAX_BYTES));
+    if (hipMemset(c->amax_dev, 0, AMAX_BYTES) != hipSuccess) {  // the shell slots start at zero
+        afd_destroy(c);
+        return fail(AFD_ERR_HIP, "hipMemset of the speed slots failed");
+    }
     TRY(dalloc(c, (void **)&c->cnt_partial, c->red_blocks * 8));
     TRY(dalloc(c, (void **)&c->cnt_out, 8));
     TRY(dalloc(c, (void **)&c->diag_partial, sizeof(double) * DIAG_N * c->red_blocks));
@@ -345,6 +363,7 @@ extern "C" int afd_set_prim(afd_ctx *c, const double *host, int64_t n) {
     if (!c || !host) return fail(AFD_ERR_INVALID, "afd_set_prim: null argument");
     if (int rc = check_n(c, n)) return rc;
     HIPCHK(hipSetDevice(c->p.device));
+    c->speed_valid = false;
     HIPCHK(hipMemcpyAsync(c->P[c->cur], host, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return AFD_OK;
@@ -657,6 +676,8 @@ static UpdLaunch upd_launch(afd_ctx *c, const double *Pin, const double *P0, dou
     a.ng = c->p.ng;
     a.gr = c->p.gr_metric == 0 ? 0 : (c->p.gr_metric == 1 ? 1 : 2);
     a.gr_x0 = c->p.gr_x0;
+    a.amax = nullptr;
+    a.sbox = c->sbox;
     a.gr_mass = c->p.gr_mass;
     a.gr_gfac = c->p.gr_metric == 0 ? 0.0 : c->p.gr_gamma / (c->p.gr_gamma - 1.0);
     a.qcap = c->c2p_qcap;
@@ -673,11 +694,16 @@ extern "C" int afd_step(afd_ctx *c, double dt) {
     const int in = c->cur, a = (in + 1) % 3, b = (in + 2) % 3;
     // SSPRK2: in -> a -> b ; SSPRK3: in -> a -> b -> a.  P[in] stays the step input.
     int src = in;
+    c->speed_valid = false;
     for (int s = 0; s < nst; ++s) {
         int dst = (s % 2 == 0) ? a : b;
         int rc = run_faces(c, c->P[src], c->fused);
         if (rc) return rc;
         UpdLaunch l = upd_launch(c, c->P[src], c->P[in], c->P[dst], s, nst, dt);
+        if (c->nb == 5 && s == nst - 1) {  // SRHD: the next step's CFL speed from the last stage
+            HIPCHK(hipMemsetAsync(c->amax_dev, 0, AMAX_BOX_BYTES, c->stream));
+            l.a.amax = c->amax_dev;
+        }
         Timed t(c, 3);
         hipError_t e;
         if (c->fused) {  // the y faces with the stage update
@@ -693,6 +719,7 @@ extern "C" int afd_step(afd_ctx *c, double dt) {
         src = dst;
     }
     c->cur = src;
+    c->speed_valid = c->nb == 5;
     c->acc.c2p_cells += (uint64_t)nst * c->g.ncell;
     return AFD_OK;
 }
@@ -716,6 +743,7 @@ extern "C" int afd_compute_rhs(afd_ctx *c, double *host, int64_t n) {
 // One pass of k_sources over the interior (the kernel applies the SRC_* bits in the driver's order).
 static int run_sources(afd_ctx *c, SourceArgs &a) {
     if (!a.which) return AFD_OK;
+    c->speed_valid = false;
     const int ng = c->p.ng;
     a.P = c->P[c->cur];
     a.g = c->g;
@@ -797,6 +825,7 @@ static bool diss_on(const afd_ctx *c, const afd_diss *d) {
 }
 
 static int run_diss(afd_ctx *c, double dt, const afd_diss *d) {
+    c->speed_valid = false;
     if (!diss_on(c, d)) return AFD_OK;
     HIPCHK(hipSetDevice(c->p.device));
     // the new fields are staged in variable planes 5..14 of a primitive buffer the step is not using
@@ -850,14 +879,29 @@ extern "C" int afd_post_step_sources(afd_ctx *c, double dt, const afd_sources *s
 extern "C" int afd_max_char_speed(afd_ctx *c, double *amax) {
     if (!c || !amax) return fail(AFD_ERR_INVALID, "afd_max_char_speed: null argument");
     HIPCHK(hipSetDevice(c->p.device));
-    MaxSpeedArgs a{c->P[c->cur], c->red_partial, c->g, c->p.gamma, c->p.gamma / (c->p.gamma - 1.0)};
+    const double gfac = c->p.gamma / (c->p.gamma - 1.0);
+    const Grid &g = c->g;
+    const SpeedBox &b = c->sbox;
+    const long range = (long)(g.nx - 2) * (g.ny - 2) * (g.nz - 2);
+    const long inner = (long)(b.hi[0] - b.lo[0]) * (b.hi[1] - b.lo[1]) * (b.hi[2] - b.lo[2]);
     {
         Timed t(c, 4);
-        hipLaunchKernelGGL(k_max_speed, dim3(c->red_blocks), dim3(BLOCK), 0, c->stream, a);
-        hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(BLOCK), 0, c->stream, (const double *)c->red_partial,
-                           c->red_blocks, SMALL, c->red_out);
+        if (!c->speed_valid) {  // no step's last stage took the box's maximum of this state
+            HIPCHK(hipMemsetAsync(c->amax_dev, 0, AMAX_BOX_BYTES, c->stream));
+            if (inner > 0)
+                hipLaunchKernelGGL(k_max_speed_box, dim3(c->red_blocks), dim3(BLOCK), 0, c->stream,
+                                   (const double *)c->P[c->cur], g, b, c->p.gamma, gfac, c->amax_dev);
+            HIPCHK(hipGetLastError());
+            c->speed_valid = true;
+        }
+        // the rest of the range as the fills left it, then the floor of the reference's maximum
+        if (range > inner)
+            hipLaunchKernelGGL(k_max_speed_shell, grid1d(range - inner), dim3(BLOCK), 0, c->stream,
+                               (const double *)c->P[c->cur], g, b, c->p.gamma, gfac,
+                               c->amax_dev + AMAX_SLOTS * AMAX_STRIDE);
+        hipLaunchKernelGGL(k_amax_finish, dim3(1), dim3(BLOCK), 0, c->stream, c->amax_dev, SMALL, c->red_out);
+        HIPCHK(hipGetLastError());
     }
-    HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(amax, c->red_out, 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return AFD_OK;

@@ -337,48 +337,71 @@ __global__ __launch_bounds__(BLOCK) void k_gr_rescale(GrRescaleArgs a) {
     for (int v = 1; v <= 3; ++v) a.Q[v * n + c] = a.P[v * n + c] / ax;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_max_speed(MaxSpeedArgs a) {
-    const Grid &g = a.g;
-    __shared__ double red[BLOCK];
-    long ni = g.nx - 2, nj = g.ny - 2, nk = g.nz - 2, tot = ni * nj * nk;
-    double amax = 0.0;
-    Phys<double> ph;
-    ph.gamma = a.gamma;
-    ph.gfac = a.gfac;
+__global__ __launch_bounds__(BLOCK) void k_max_speed_box(const double *P, Grid g, SpeedBox b, double gamma,
+                                                          double gfac, unsigned long long *amax) {
+    const long ni = b.hi[0] - b.lo[0], nj = b.hi[1] - b.lo[1], nk = b.hi[2] - b.lo[2], tot = ni * nj * nk;
+    double m = 0.0;
     for (long t = (long)blockIdx.x * BLOCK + threadIdx.x; t < tot; t += (long)gridDim.x * BLOCK) {
-        int k = 1 + (int)(t % nk), j = 1 + (int)((t / nk) % nj), i = 1 + (int)(t / (nj * nk));
-        long c = cidx(g, i, j, k);
-        double rho = a.P[c], vx = a.P[g.ncell + c], vy = a.P[2 * g.ncell + c], vz = a.P[3 * g.ncell + c],
-               p = a.P[4 * g.ncell + c];
-        double cs = srhd_sound_speed(ph, rho, p);
-        double l0, l1, l2, l3, l4, l5;
-        eig_speeds(vx, cs, l0, l1);
-        eig_speeds(vy, cs, l2, l3);
-        eig_speeds(vz, cs, l4, l5);
-        double loc = pymax(pymax(pymax(pymax(pymax(fabs(l0), fabs(l1)), fabs(l2)), fabs(l3)), fabs(l4)), fabs(l5));
-        if (loc > amax) amax = loc;
+        const int k = b.lo[2] + (int)(t % nk), j = b.lo[1] + (int)((t / nk) % nj), i = b.lo[0] + (int)(t / (nj * nk));
+        const long c = cidx(g, i, j, k);
+        const double loc = srhd_cell_speed(gamma, gfac, P[c], P[g.ncell + c], P[2 * g.ncell + c],
+                                           P[3 * g.ncell + c], P[4 * g.ncell + c]);
+        if (loc > m) m = loc;
     }
-    red[threadIdx.x] = amax;
-    __syncthreads();
-    for (int o = BLOCK / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] = pymax(red[threadIdx.x], red[threadIdx.x + o]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) a.partial[blockIdx.x] = red[0];
+    wave_max_atomic(amax, m);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_reduce_max(const double *partial, int n, double floor_value,
-                                                       double *out) {
-    __shared__ double red[BLOCK];
-    double m = 0.0;
-    for (int t = threadIdx.x; t < n; t += BLOCK) m = pymax(m, partial[t]);
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int o = BLOCK / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] = pymax(red[threadIdx.x], red[threadIdx.x + o]);
-        __syncthreads();
+__global__ __launch_bounds__(BLOCK) void k_max_speed_shell(const double *P, Grid g, SpeedBox b, double gamma,
+                                                            double gfac, unsigned long long *amax) {
+    // range [1, n-1) per axis, box [lo, hi) inside it: slabs i outside (all j, k), then i inside
+    // with j outside (all k), then i, j inside with k outside
+    const long bi = g.nx - 2, bj = g.ny - 2, bk = g.nz - 2;
+    const long ni = b.hi[0] - b.lo[0], nj = b.hi[1] - b.lo[1], nk = b.hi[2] - b.lo[2];
+    const long oi = bi - ni, oj = bj - nj, ok = bk - nk;
+    const long nA = oi * bj * bk, nB = ni * oj * bk, nC = ni * nj * ok;
+    long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    double v = 0.0;
+    if (t < nA + nB + nC) {
+        auto out = [&](long p, int ax) { return p < b.lo[ax] - 1 ? 1 + p : b.hi[ax] + (p - (b.lo[ax] - 1)); };
+        long i, j, k;
+        if (t < nA) {
+            k = 1 + t % bk;
+            j = 1 + (t / bk) % bj;
+            i = out(t / (bk * bj), 0);
+        } else if (t < nA + nB) {
+            t -= nA;
+            k = 1 + t % bk;
+            j = out((t / bk) % oj, 1);
+            i = b.lo[0] + t / (bk * oj);
+        } else {
+            t -= nA + nB;
+            k = out(t % ok, 2);
+            j = b.lo[1] + (t / ok) % nj;
+            i = b.lo[0] + t / (ok * nj);
+        }
+        const long c = cidx(g, (int)i, (int)j, (int)k);
+        v = srhd_cell_speed(gamma, gfac, P[c], P[g.ncell + c], P[2 * g.ncell + c], P[3 * g.ncell + c],
+                            P[4 * g.ncell + c]);
     }
-    if (threadIdx.x == 0) out[0] = red[0] < floor_value ? floor_value : red[0];
+    wave_max_atomic(amax, v);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_amax_finish(unsigned long long *amax, double floor_value, double *out) {
+    __shared__ double red[BLOCK / 64];
+    double m = 0.0;
+    for (int t = threadIdx.x; t < 2 * AMAX_SLOTS; t += BLOCK) {
+        unsigned long long *slot = amax + (long)t * AMAX_STRIDE;
+        m = pymax(m, __longlong_as_double((long long)*slot));
+        if (t >= AMAX_SLOTS) *slot = 0ull;  // each shell slot is read and cleared by this thread only
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = pymax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < BLOCK / 64; ++w) m = pymax(m, red[w]);
+        out[0] = m < floor_value ? floor_value : m;
+    }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_count_nonfinite(const double *x, long n, unsigned long long *partial) {

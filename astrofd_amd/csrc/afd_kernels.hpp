@@ -745,6 +745,12 @@ AFD_D double passive_flux(const PassiveArgs &a, int variant, int v, int dir, int
 __global__ void k_passive_rhs(PassiveArgs a);
 
 // ------------------------------------------------------------------ stage update + c2p
+// Cells [lo, hi) per axis whose CFL speed a step's last stage takes: the flux range inside the
+// interior [ng, n - ng), the part of max_char_speed's range [1, n-1)^3 that no fill writes.
+struct SpeedBox {
+    int lo[3], hi[3];
+};
+
 template <typename F> struct UpdArgs {
     const double *P0;     // step input primitives (U0 = p2c(P0) at every stage, passives t0)
     const double *Pin;    // stage input primitives (GLM, resistivity, passives t_{s-1})
@@ -766,7 +772,42 @@ template <typename F> struct UpdArgs {
     long qcap;
     int ng, gr, gr_x0;            // GR sources: 0 none, 1 isotropic-Schwarzschild lapse, 2 Kerr-Schild
     double gr_mass, gr_gfac;      // GR_MASS; Gamma / (Gamma - 1) of the sources' enthalpy
+    unsigned long long *amax;     // SRHD last stage: max characteristic speed over sbox of the new
+                                  //   primitives, as the bits of a double >= 0 (atomicMax into
+                                  //   AMAX_SLOTS slots), or null
+    SpeedBox sbox;
 };
+
+// core/srhd_core.py:1268-1281 (max_char_speed): the largest |eigenvalue| of one cell's primitives
+AFD_D double srhd_cell_speed(double gamma, double gfac, double rho, double vx, double vy, double vz, double p) {
+    Phys<double> ph;
+    ph.gamma = gamma;
+    ph.gfac = gfac;
+    const double cs = srhd_sound_speed(ph, rho, p);
+    double l0, l1, l2, l3, l4, l5;
+    eig_speeds(vx, cs, l0, l1);
+    eig_speeds(vy, cs, l2, l3);
+    eig_speeds(vz, cs, l4, l5);
+    return pymax(pymax(pymax(pymax(pymax(fabs(l0), fabs(l1)), fabs(l2)), fabs(l3)), fabs(l4)), fabs(l5));
+}
+
+// The speed maximum is kept in AMAX_SLOTS slots AMAX_STRIDE words (256 B) apart, one per
+// blockIdx % AMAX_SLOTS: one address taking every wave's atomic serialises them (a 256^3 update
+// runs ~5x slower), spread slots run in parallel.
+constexpr int AMAX_SLOTS = 256, AMAX_STRIDE = 32;
+constexpr size_t AMAX_BOX_BYTES = sizeof(unsigned long long) * AMAX_SLOTS * AMAX_STRIDE;
+constexpr size_t AMAX_BYTES = 2 * AMAX_BOX_BYTES;  // the speed box's slots, then the shell's
+
+// The wave's largest v (every lane takes part; v >= 0, NaN counts as 0, as in the reduction of
+// k_max_speed, which never takes a NaN) into the block's slot of dst by one atomicMax on the bits
+// of a double.
+AFD_D void wave_max_atomic(unsigned long long *dst, double v) {
+    if (!(v > 0.0)) v = 0.0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = pymax(v, __shfl_xor(v, o));
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(dst + (blockIdx.x % AMAX_SLOTS) * AMAX_STRIDE, (unsigned long long)__double_as_longlong(v));
+}
 
 // Rejected cells of the learned guess are queued per workgroup slot (blockIdx % QUEUE_SLOTS),
 // each slot a region of qcap = ceil(blocks / QUEUE_SLOTS) * BLOCK entries, so no slot overflows
@@ -1065,6 +1106,23 @@ AFD_D void cell_update(const UpdArgs<F> &a, long c, bool live, const double *rhs
         }
     }
     if (a.stats) wave_add_stats(a.stats, st);
+    if constexpr (NB == 5) {
+        // the next step's CFL speed over the speed box, from the primitives just stored (the
+        // fills after the step write only outside it; k_max_speed_shell takes the rest of
+        // max_char_speed's range after them), so that afd_max_char_speed needs no pass over the
+        // whole state
+        if (a.amax) {
+            double v = 0.0;
+            if (live) {
+                int i, j, k;
+                cell_ijk(g, c, i, j, k);
+                const SpeedBox &b = a.sbox;
+                if (i >= b.lo[0] && i < b.hi[0] && j >= b.lo[1] && j < b.hi[1] && k >= b.lo[2] && k < b.hi[2])
+                    v = srhd_cell_speed(a.gamma, a.gfac, q[0], q[1], q[2], q[3], q[4]);
+            }
+            wave_max_atomic(a.amax, v);
+        }
+    }
 }
 
 // Stage update of one cell per thread: flux divergence + sources, then cell_update.  LEARNED
@@ -1329,17 +1387,17 @@ __global__ void k_gr_rescale(GrRescaleArgs a);
 __global__ void k_diss_store(DissArgs a, double *P);
 
 // ------------------------------------------------------------------ reductions
-// core/srhd_core.py:1267-1281 over cells [1, n-1)^3; one partial maximum per block
-struct MaxSpeedArgs {
-    const double *P;
-    double *partial;
-    Grid g;
-    double gamma, gfac;
-};
-__global__ void k_max_speed(MaxSpeedArgs a);
-
-__global__ void k_reduce_max(const double *partial, int n, double floor_value,
-                                                       double *out);
+// core/srhd_core.py:1267-1281, the maximum over cells [1, n-1)^3, in two parts: the cells of the
+// speed box (SpeedBox: the flux range inside the interior, which the fills never write), taken by
+// the last stage of a step (cell_update) or by k_max_speed_box into the first AMAX_SLOTS slots,
+// and the rest, taken by k_max_speed_shell after the fills into the next AMAX_SLOTS slots.  Both
+// atomicMax the bits of a double >= 0.  k_amax_finish (one block of BLOCK threads) writes
+// max(all slots, floor) as a double to out and zeroes the shell's slots for the next call.
+__global__ void k_max_speed_box(const double *P, Grid g, SpeedBox b, double gamma, double gfac,
+                                unsigned long long *amax);
+__global__ void k_max_speed_shell(const double *P, Grid g, SpeedBox b, double gamma, double gfac,
+                                  unsigned long long *amax);
+__global__ void k_amax_finish(unsigned long long *amax, double floor_value, double *out);
 
 // Diagnostics of utils/diagnostics.py over the interior [ng, ng + n) per axis: maximum Lorentz
 // factor (:15-27), inlet flux sum of Sx dy dz on the plane i = ng (:31-43; the reference's

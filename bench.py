@@ -125,7 +125,10 @@ def run_timed(s, pr0, steps, warmup, mlp=None, repeat=True, face_path=0):
 
     repeat=True: every step starts from the seeded initial state (re-uploaded and filled
     outside the timed interval), so each timed step is one pass of the hot path over the same
-    synthetic input.  repeat=False: consecutive steps of one run."""
+    synthetic input.  SRHD: the reset also leaves the input's CFL maximum over the speed box, as
+    a run's previous step does, so the timed step does a run's step's CFL work (the ghost shell
+    before it, the next step's box maximum in its last stage).  repeat=False: consecutive steps
+    of one run."""
     d, rng = open_domain(s, pr0, mlp, face_path)
     try:
         one_step = make_stepper(d, s, rng)
@@ -134,6 +137,11 @@ def run_timed(s, pr0, steps, warmup, mlp=None, repeat=True, face_path=0):
             if repeat:
                 d.set_prim(pr0)
                 d.apply_bcs(rng)
+                if s["PHYSICS"] in ("hydro", "grhd"):
+                    # the speed box's maximum of the input, as the previous step's last stage leaves
+                    # it in a run; the timed step adds the ghost shell and takes the next step's
+                    # maximum in its own last stage
+                    d.max_char_speed()
 
         for _ in range(warmup):
             reset()

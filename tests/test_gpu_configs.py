@@ -191,3 +191,43 @@ def test_rmhd_config_as_shipped(name, n):
         log(f"      {precision}/{guess}: L1 rho {l1s[0]:.2e} p {l1s[4]:.2e} B {l1s[5]:.2e} {l1s[6]:.2e} {l1s[7]:.2e}; "
             f"ratio to spread max {float((l1s / np.maximum(spread, 1e-300)).max()):.2f}")
         assert np.all(l1s <= 10.0 * spread + 1e-12)
+
+
+@pytest.mark.parametrize("recon,ng", [("muscl", 2), ("muscl", 3), ("ppm", 4)])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_srhd_cfl_speed_from_the_step(precision, recon, ng):
+    """afd_max_char_speed after a step: the last stage's update took the maximum over the speed box
+    (the flux range inside the interior) and k_max_speed_shell takes the rest of max_char_speed's
+    range [1, n-1)^3 as the fills left it.  Checked against the oracle's maximum of the same
+    primitives, before and after the fills (the shell's slots must not keep the stale ghosts),
+    and bitwise against the box kernel's maximum of the same state (set_prim drops the step's
+    value).  NG 3 (MUSCL) and 4 (PPM) put the flux range's first cells in the ghost layers."""
+    import oracle as O
+    log(f"test_srhd_cfl_speed_from_the_step {precision} {recon} NG={ng}")
+    s = dict(settings("srhd_jet_32", 24), PRECISION=precision, RECON=recon, NG=ng)
+    d = Domain.from_settings(s)
+    pr0 = ic.initial_state(s)
+    P = oracle_loop.params_from_settings(s, pr0.shape)
+    try:
+        d.set_prim(pr0)
+        d.set_nozzle(s)
+        d.apply_bcs()
+        dx = min(s["Lx"] / s["NX"], s["Ly"] / s["NY"], s["Lz"] / s["NZ"])
+        for n in range(4):
+            a_step = d.max_char_speed()
+            pr = d.get_prim()
+            a_ref = O.max_char_speed(P, pr)
+            d.set_prim(pr)  # the same state; the next call takes the box kernel
+            a_box = d.max_char_speed()
+            log(f"  step {n}: {a_step!r} {a_box!r} oracle {a_ref!r}")
+            assert a_step == a_box
+            assert abs(a_step - a_ref) <= 1e-15
+            dt = s["CFL"] * dx / max(a_box, 1e-12)
+            d.apply_bcs()
+            d.step(dt)
+            a_stale = d.max_char_speed()  # ghosts as the step left them
+            assert abs(a_stale - O.max_char_speed(P, d.get_prim())) <= 1e-15
+            d.apply_bcs()
+            d.post_step_sources(dt)
+    finally:
+        d.close()
