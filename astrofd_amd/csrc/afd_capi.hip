@@ -585,6 +585,7 @@ static FaceLaunch face_launch(afd_ctx *c, const double *P) {
     l.mixed = c->p.precision == AFD_PREC_MIXED;
     l.shared_edges = c->shared_edges;
     l.D = c->dmode ? (float *)c->Fx : nullptr;
+    l.dfirst = true;
     l.idn = 0.0f;
     return l;
 }
@@ -599,11 +600,16 @@ static int run_faces(afd_ctx *c, const double *P, bool skip_y = false) {
     }
     FaceLaunch l = face_launch(c, P);
     void *Fd[3] = {c->Fx, c->Fy, c->Fz};
-    const int order_carried[3] = {2, 0, 1}, order_plain[3] = {0, 1, 2};
-    const int *order = c->carried ? order_carried : order_plain;
+    // carried x speeds: z before x.  Flux divergence: the first launch writes D, so y (when
+    // launched here) goes first
+    const int order_carried[3] = {2, 0, 1}, order_plain[3] = {0, 1, 2}, order_div[3] = {1, 2, 0};
+    const int *order = c->dmode && !skip_y ? order_div : (c->carried ? order_carried : order_plain);
+    bool first = true;
     for (int n = 0; n < 3; ++n) {
         int d = order[n];
         if (skip_y && d == 1) continue;  // the stage update evaluates the y faces itself
+        l.dfirst = first;
+        first = false;
         l.dir = d;
         l.idn = (float)(1.0 / (d == 0 ? c->p.dx : (d == 1 ? c->p.dy : c->p.dz)));
         l.F = Fd[d];

@@ -532,9 +532,11 @@ void k_face_rmhd_weno_lanes(FaceArgs<float> a) {
 // ------------------------------------------------------------------ RMHD WENO5 flux divergence, FP32 path
 // The mixed path's shared-edge WENO5 kernels accumulate each cell's flux divergence
 // D = sum_d (F(right face) - F(left face)) / d_d in FP32 (nb x ncell) instead of storing face
-// fluxes: the z kernel writes D, then the x and y kernels add their parts, and the stage update
-// reads one FP32 array (36 B a cell) where it read four flux arrays (144 B).  The z kernel still
-// stores the z-back speeds the x kernel's carried fluxes need.
+// fluxes: the first kernel of a stage writes D (ADD false), the others add their parts, and the
+// stage update reads one FP32 array (36 B a cell) where it read four flux arrays (144 B).  The
+// order is y, z, x (the y pencils write, so their rounds wait on no read of D; the z lanes,
+// independent waves, hide the read's latency), or z, x when the stage update evaluates the y
+// faces.  The z kernel also stores the z-back speeds the x kernel's carried fluxes need.
 
 // The Riemann flux(es) of x or y face (i, j, k) (between cells c - s and c): Fown with the face's
 // own speeds (the right face of cell c - s; wanted when own), Fcar with the speeds carried from
@@ -568,7 +570,7 @@ AFD_D void rmhd_face_pair(const FaceArgs<float> &a, const RmhdFace<DIR, float> &
 #define AFD_LANES_DIV_WAVES AFD_FACE_WAVES_F32
 #endif
 
-template <int RIEM>
+template <int RIEM, bool ADD>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(AFD_LANES_DIV_WAVES)))
 void k_face_rmhd_weno_lanes_div(FaceArgs<float> a) {
     const Grid &g = a.g;
@@ -582,6 +584,9 @@ void k_face_rmhd_weno_lanes_div(FaceArgs<float> a) {
     const unsigned line = udiv(a.dk, pos), kk = pos - line * ncz, ii = udiv(a.dn, line);
     const int i = g.r0[0] + (int)ii, j = g.r0[1] + (int)(line - ii * ny), k = g.r0[2] - 1 + (int)kk;
     const unsigned c = (unsigned)cidx(g, i, j, k), n = (unsigned)g.ncell;
+    float dold[9];  // ADD: the cell's divergence so far, loaded with the stencil (c is in range)
+#pragma unroll
+    for (int v = 0; v < 9; ++v) dold[v] = ADD ? a.D[v * n + c] : 0.0f;
     State9 L, R;
     float lo[9], hi[9];
     weno5_cell_edges9(a.P, n, c, 1u, L, R, lo, hi);
@@ -602,7 +607,7 @@ void k_face_rmhd_weno_lanes_div(FaceArgs<float> a) {
     a.zs[n + c] = fc.sR;
     if (kk + 2u > (unsigned)ncz) return;  // the face k = r1 closes a line: no cell of its own
 #pragma unroll
-    for (int v = 0; v < 9; ++v) a.D[v * n + c] = (Fr[v] - F[v]) * a.idn;
+    for (int v = 0; v < 9; ++v) a.D[v * n + c] = ADD ? dold[v] + (Fr[v] - F[v]) * a.idn : (Fr[v] - F[v]) * a.idn;
 }
 
 // x and y faces as pencils: a workgroup of four waves covers 64 consecutive z positions of one
@@ -623,7 +628,7 @@ constexpr int MARCH_CELLS = AFD_MARCH_CELLS;
 #define AFD_MARCH_WAVES AFD_FACE_WAVES_F32
 #endif
 
-template <int RIEM, int DIR>
+template <int RIEM, int DIR, bool ADD>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AFD_MARCH_WAVES)))
 void k_face_rmhd_weno_march(FaceArgs<float> a) {
     static_assert(DIR == 0 || DIR == 1, "pencils along x or y");
@@ -652,12 +657,13 @@ void k_face_rmhd_weno_march(FaceArgs<float> a) {
         const bool right_of = eidx >= 2 && eidx < count, left_of = eidx >= 1 && eidx <= count - 2;
         const bool upd = lane_ok && right_of;
         const unsigned cl = c - s;
-        // x: cell e - 1's divergence so far, loaded before the face arithmetic (1.42 -> 1.21 ms a
-        // launch at 256^3); loaded at the end as in y, the loads' latency showed at every round.
-        // y keeps the late load (1.05 ms; 1.42 ms loaded early)
+        // ADD, x: cell e - 1's divergence so far, loaded before the face arithmetic (1.42 -> 1.21
+        // ms a launch at 256^3); loaded at the end, the loads' latency showed at every round.  In
+        // y, loaded early it cost registers (1.42 ms), loaded late its latency (1.05 ms); y now
+        // runs first and only writes
         float dold[9];
 #pragma unroll
-        for (int v = 0; v < 9; ++v) dold[v] = (DIR == 0 && upd) ? a.D[v * n + cl] : 0.0f;
+        for (int v = 0; v < 9; ++v) dold[v] = (ADD && DIR == 0 && upd) ? a.D[v * n + cl] : 0.0f;
         State9 L, R;
         float lo[9], hi[9];
         weno5_cell_edges9(a.P, n, c, s, L, R, lo, hi);
@@ -688,7 +694,8 @@ void k_face_rmhd_weno_march(FaceArgs<float> a) {
 #pragma unroll
             for (int v = 0; v < 9; ++v) {
                 const float fl = w > 0 ? flx[buf][w - 1][v][lane] : flx[buf ^ 1][3][v][lane];
-                a.D[v * n + cl] = (DIR == 0 ? dold[v] : a.D[v * n + cl]) + (Fown[v] - fl) * a.idn;
+                const float d = (Fown[v] - fl) * a.idn;
+                a.D[v * n + cl] = ADD ? (DIR == 0 ? dold[v] : a.D[v * n + cl]) + d : d;
             }
         }
     }

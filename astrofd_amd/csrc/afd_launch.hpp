@@ -19,6 +19,7 @@ struct FaceLaunch {
     bool mixed;
     bool shared_edges;  // FP32 RMHD WENO5: the shared-edge kernels (afd_params.face_path == 0)
     float *D;           // FP32 RMHD WENO5 with shared edges: the flux-divergence accumulator (else null)
+    bool dfirst;        // ... written by this launch (the stage's first face kernel), not added to
     float idn;          // 1 / spacing along dir
 };
 

@@ -42,7 +42,8 @@ template <int RIEM> static void go_weno_z_div(const FaceLaunch &l, hipStream_t s
     const unsigned ncz = nz + 2, npos = ncz * ny * nx, waves = (npos - 1 + 61) / 62;
     a.dk = make_udiv(ncz);
     a.dn = make_udiv(ny);
-    hipLaunchKernelGGL((k_face_rmhd_weno_lanes_div<RIEM>), grid1d(64L * waves), dim3(BLOCK), 0, s, a);
+    if (l.dfirst) hipLaunchKernelGGL((k_face_rmhd_weno_lanes_div<RIEM, false>), grid1d(64L * waves), dim3(BLOCK), 0, s, a);
+    else hipLaunchKernelGGL((k_face_rmhd_weno_lanes_div<RIEM, true>), grid1d(64L * waves), dim3(BLOCK), 0, s, a);
 }
 
 // FP32 WENO5 flux divergence, x and y: four-wave pencils sweeping runs of MARCH_CELLS cells.
@@ -55,8 +56,11 @@ template <int RIEM> static void go_weno_xy_march(const FaceLaunch &l, hipStream_
     a.dk = make_udiv(nkc);
     a.dn = make_udiv(nruns);
     const dim3 grid(nkc * nruns * no), block(256);
-    if (l.dir == 0) hipLaunchKernelGGL((k_face_rmhd_weno_march<RIEM, 0>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_face_rmhd_weno_march<RIEM, 1>), grid, block, 0, s, a);
+    // x always adds (the z kernel runs before it: its carried fluxes need the z-back speeds); y
+    // writes on the flux-divergence path, where it runs first
+    if (l.dir == 0) hipLaunchKernelGGL((k_face_rmhd_weno_march<RIEM, 0, true>), grid, block, 0, s, a);
+    else if (l.dfirst) hipLaunchKernelGGL((k_face_rmhd_weno_march<RIEM, 1, false>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_face_rmhd_weno_march<RIEM, 1, true>), grid, block, 0, s, a);
 }
 
 template <int RECON, int RIEM, typename T> static void go(const FaceLaunch &l, hipStream_t s) {
