@@ -1074,6 +1074,11 @@ AFD_D void cell_update(const UpdArgs<F> &a, long c, bool live, const double *rhs
         double q0[NB];
 #pragma unroll
         for (int v = 0; v < NB; ++v) q0[v] = a.P0[v * g.ncell + c];
+        // every Uc load before the first Uc store: the compiler cannot tell the variables' planes
+        // apart, and would otherwise wait on each load after the previous plane's store
+        double uc[NB];
+#pragma unroll
+        for (int v = 0; v < NB; ++v) uc[v] = (a.stage == 0) ? 0.0 : a.Uc[v * g.ncell + c];
         if constexpr (NB == 9) rmhd_p2c_cell(a.gfac, a.p_max, a.v_max, q0, U);
         else srhd_p2c_cell(a.gfac, q0, U);
 #pragma unroll
@@ -1083,8 +1088,7 @@ AFD_D void cell_update(const UpdArgs<F> &a, long c, bool live, const double *rhs
 #pragma clang fp contract(off)
                 r = a.dt * rhs[v];
             }
-            double uc = (a.stage == 0) ? 0.0 : a.Uc[v * g.ncell + c];
-            U[v] = ssprk_combine(a.stage, a.nst, U[v], uc, r);
+            U[v] = ssprk_combine(a.stage, a.nst, U[v], uc[v], r);
             if (!last) a.Uc[v * g.ncell + c] = U[v];  // the last stage's U is read only by k_c2p_queue
         }
     }
