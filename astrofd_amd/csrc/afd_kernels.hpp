@@ -1020,7 +1020,7 @@ AFD_D double prim_Z(double gfac, double rho, double vx, double vy, double vz, do
 // wave-wide); `live` marks lanes with a cell.  Returns whether q holds the cell's primitives.
 template <int NB, bool LEARNED, typename F, bool LDSW = false>
 AFD_D bool cell_c2p(const UpdArgs<F> &a, const double *U, bool live, long c, double *q, C2PStats &st,
-                    const MlpLds *lw) {
+                    MlpLds *lw, const MlpStaged<BLOCK> *ws) {
     // The mixed RMHD con2prim uses the fast FP64 rcp/rsq: its hydro stage only seeds a Newton solve
     // accepted by its residual.  The SRHD con2prim's output is the fixed point's own branch (converged
     // or the energy fallback), which ulp-level changes can flip, so it stays IEEE.
@@ -1031,10 +1031,12 @@ AFD_D bool cell_c2p(const UpdArgs<F> &a, const double *U, bool live, long c, dou
         if constexpr (LEARNED) {
             float x[5], y[2];
             bool fok = mlp_features(U, x) && live;
-            if constexpr (LDSW)  // the workgroup's LDS copy of the weights
+            if constexpr (LDSW) {  // the workgroup's LDS copy of the weights
+                if (ws) mlp_stage_store<BLOCK>(*ws, *lw, threadIdx.x);  // loaded at the kernel's start
                 mlp_infer_wave(MlpShared{lw}, x, y);
-            else
+            } else {
                 mlp_infer_wave(MlpGlobal{mlp_view(a.mlp)}, x, y);
+            }
             if (fok) done = rmhd_c2p_from_guess<FAST>(cp, U, mlp_guess_Z(U[0], a.gfac, y), q, st);
         } else {
             if (a.guess == 2 && live) {
@@ -1060,7 +1062,8 @@ AFD_D bool cell_c2p(const UpdArgs<F> &a, const double *U, bool live, long c, dou
 // Pout.  Reached by every lane of the wave (the learned start runs wave-wide).  LDSW: the
 // network's forward weights come from the workgroup's LDS copy lw.
 template <int NB, typename F, bool LEARNED, bool LDSW = false>
-AFD_D void cell_update(const UpdArgs<F> &a, long c, bool live, const double *rhs, const MlpLds *lw = nullptr) {
+AFD_D void cell_update(const UpdArgs<F> &a, long c, bool live, const double *rhs, MlpLds *lw = nullptr,
+                       const MlpStaged<BLOCK> *ws = nullptr) {
     const Grid &g = a.g;
     C2PStats st = {0, 0, 0};
     double U[NB], q[NB];
@@ -1092,7 +1095,7 @@ AFD_D void cell_update(const UpdArgs<F> &a, long c, bool live, const double *rhs
             if (!last) a.Uc[v * g.ncell + c] = U[v];  // the last stage's U is read only by k_c2p_queue
         }
     }
-    const bool have = cell_c2p<NB, LEARNED, F, LDSW>(a, U, live, c, q, st, lw);
+    const bool have = cell_c2p<NB, LEARNED, F, LDSW>(a, U, live, c, q, st, lw, ws);
     if (live) {
         if (have) {
 #pragma unroll
@@ -1155,11 +1158,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LEARNED ?
     const long c = (long)blockIdx.x * BLOCK + threadIdx.x;
     const bool live = c < g.ncell;
     if constexpr (LEARNED) {
-        __shared__ MlpLds mw;  // the network's forward weights, read by every wave of the workgroup
-        mlp_stage_lds<BLOCK>(a.mlp, mw, threadIdx.x);
+        // the network's forward weights, read by every wave of the workgroup from LDS: loaded
+        // first, written to LDS just before the network runs, so that their round trip overlaps
+        // the cell's own loads
+        __shared__ MlpLds mw;
+        MlpStaged<BLOCK> ws;
+        mlp_stage_load<BLOCK>(a.mlp, ws, threadIdx.x);
         double rhs[NB];
         if (live) cell_rhs<NB, F>(a, c, in_range(g, c), rhs);
-        cell_update<NB, F, true, true>(a, c, live, rhs, &mw);
+        cell_update<NB, F, true, true>(a, c, live, rhs, &mw, &ws);
     } else {
         double rhs[NB];
         if (live) cell_rhs<NB, F>(a, c, in_range(g, c), rhs);

@@ -76,29 +76,45 @@ struct MlpShared {
     AFD_D float b3(int i) const { return s->b3[i]; }
 };
 
-// Copies the forward blocks and b3 of the packed weights w to s.  Called by every thread of a
-// workgroup of NT threads (tid in [0, NT)); ends with a barrier.  The loads are all issued before
-// the first store, so that a workgroup waits for one memory round trip, not one per copy.
-template <int NT> AFD_D void mlp_stage_lds(const void *w, MlpLds &s, int tid) {
+// Copies the forward blocks and b3 of the packed weights w to s, in two halves: mlp_stage_load
+// issues a thread's loads into registers, mlp_stage_store writes them to LDS and ends with a
+// barrier.  Called by every thread of a workgroup of NT threads (tid in [0, NT)).  The loads are
+// all issued before the first store, so that a workgroup waits for one memory round trip, not
+// one per copy; between the halves the workgroup can issue its own loads, whose round trip then
+// overlaps the weights'.
+template <int NT> struct MlpStaged {
     static_assert((MLP_FWD_BLK * 64) % NT == 0, "whole copies per thread");
-    constexpr int PER = MLP_FWD_BLK * 64 / NT;
+    static constexpr int PER = MLP_FWD_BLK * 64 / NT;
+    bf16x8 v[PER];
+    float b3;
+};
+
+template <int NT> AFD_D void mlp_stage_load(const void *w, MlpStaged<NT> &st, int tid) {
     const bf16x8 *g = (const bf16x8 *)w;
     const float *bias = (const float *)((const char *)w + (size_t)MLP_NBLK * 64 * 16);
-    bf16x8 v[PER];
 #pragma unroll
-    for (int r = 0; r < PER; ++r) {
+    for (int r = 0; r < MlpStaged<NT>::PER; ++r) {
         const int i = tid + r * NT, slot = i >> 6, l = i & 63;
         const int b = slot < 14 ? slot : BLK_B2 + (slot - 14);
-        v[r] = g[b * 64 + l];
+        st.v[r] = g[b * 64 + l];
     }
-    const float b3 = tid < 2 ? bias[128 + tid] : 0.0f;
+    st.b3 = tid < 2 ? bias[128 + tid] : 0.0f;
+}
+
+template <int NT> AFD_D void mlp_stage_store(const MlpStaged<NT> &st, MlpLds &s, int tid) {
 #pragma unroll
-    for (int r = 0; r < PER; ++r) {
+    for (int r = 0; r < MlpStaged<NT>::PER; ++r) {
         const int i = tid + r * NT;
-        s.blk[i >> 6][i & 63] = v[r];
+        s.blk[i >> 6][i & 63] = st.v[r];
     }
-    if (tid < 2) s.b3[tid] = b3;
+    if (tid < 2) s.b3[tid] = st.b3;
     __syncthreads();
+}
+
+template <int NT> AFD_D void mlp_stage_lds(const void *w, MlpLds &s, int tid) {
+    MlpStaged<NT> st;
+    mlp_stage_load<NT>(w, st, tid);
+    mlp_stage_store<NT>(st, s, tid);
 }
 
 AFD_D floatx16 mfma_bf16(bf16x8 a, bf16x8 b, floatx16 c) {
