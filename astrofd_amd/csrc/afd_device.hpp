@@ -383,19 +383,24 @@ template <bool FAST = false> AFD_D double rmhd_cap_W(double v_max, const double 
 }
 
 // core/rmhd_core.py:137-168; vc = capped velocity, W = its Lorentz factor, q = state
-template <typename T> AFD_D void rmhd_p2c_W(const Phys<T> &ph, const T *q, const T *vc, T W, T *U) {
+// FAST (FP64 only): the divisions by ddiv<true>, as the mixed path's con2prim
+template <typename T, bool FAST = false> AFD_D void rmhd_p2c_W(const Phys<T> &ph, const T *q, const T *vc, T W, T *U) {
+    auto dv = [](T a, T b) {
+        if constexpr (sizeof(T) == 8) return ddiv<FAST>(a, b);
+        else return fdiv(a, b);
+    };
     T rho = q[0], p = q[4], Bx = q[5], By = q[6], Bz = q[7];
     T vx = vc[0], vy = vc[1], vz = vc[2];
     if (p < Lim<T>::small) p = Lim<T>::small;
     if (p > ph.p_max) p = ph.p_max;
     T B2 = Bx * Bx + By * By + Bz * Bz;
-    T h = T(1) + fdiv(ph.gfac * p, pymax(rho, Lim<T>::small));
+    T h = T(1) + dv(ph.gfac * p, pymax(rho, Lim<T>::small));
     T rhoh = rho * h;
     T vb = vx * Bx + vy * By + vz * Bz;
     T b0 = W * vb;
-    T bx = fdiv(Bx + b0 * vx, W);
-    T by = fdiv(By + b0 * vy, W);
-    T bz = fdiv(Bz + b0 * vz, W);
+    T bx = dv(Bx + b0 * vx, W);
+    T by = dv(By + b0 * vy, W);
+    T bz = dv(Bz + b0 * vz, W);
     T b2 = (bx * bx + by * by + bz * bz) - b0 * b0;
     T rhohWW = rhoh * W * W;
     T wtot = rhohWW + B2;
@@ -863,13 +868,15 @@ AFD_D void srhd_p2c_cell(double gfac, const double *q, double *U) {
     ph.gfac = gfac;
     srhd_p2c_W(ph, q, W, U);
 }
+// FAST: the Lorentz factor and the divisions by the hardware reciprocals (the mixed path's U0)
+template <bool FAST = false>
 AFD_D void rmhd_p2c_cell(double gfac, double p_max, double v_max, const double *q, double *U) {
     double vc[3];
-    double W = rmhd_cap_W(v_max, q, vc);
+    double W = rmhd_cap_W<FAST>(v_max, q, vc);
     Phys<double> ph;
     ph.gfac = gfac;
     ph.p_max = p_max;
-    rmhd_p2c_W(ph, q, vc, W, U);
+    rmhd_p2c_W<double, FAST>(ph, q, vc, W, U);
 }
 
 }  // namespace afd

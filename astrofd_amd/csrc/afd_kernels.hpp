@@ -1082,7 +1082,9 @@ AFD_D void cell_update(const UpdArgs<F> &a, long c, bool live, const double *rhs
         double uc[NB];
 #pragma unroll
         for (int v = 0; v < NB; ++v) uc[v] = (a.stage == 0) ? 0.0 : a.Uc[v * g.ncell + c];
-        if constexpr (NB == 9) rmhd_p2c_cell(a.gfac, a.p_max, a.v_max, q0, U);
+        // the mixed path takes the hardware reciprocals here as in its con2prim: U0 moves by a
+        // few ulp, far below the FP32 flux rounding of the same stage
+        if constexpr (NB == 9) rmhd_p2c_cell<sizeof(F) == 4>(a.gfac, a.p_max, a.v_max, q0, U);
         else srhd_p2c_cell(a.gfac, q0, U);
 #pragma unroll
         for (int v = 0; v < NB; ++v) {
