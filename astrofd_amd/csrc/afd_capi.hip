@@ -81,6 +81,8 @@ struct afd_ctx {
     MlpDevice mlp{};
     int *c2p_queue = nullptr;         // cells whose learned start failed the acceptance test (QUEUE_SLOTS regions)
     unsigned *c2p_qcount = nullptr;
+    int *repair_queue = nullptr;      // flux-divergence path: cells whose FP32 right-hand side was not finite
+    unsigned *repair_qcount = nullptr;
     long c2p_qcap = 0;
     std::vector<void *> allocs;
 };
@@ -310,7 +312,7 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
     TRY(dalloc(c, (void **)&c->diag_out, sizeof(double) * DIAG_N));
     TRY(dalloc(c, (void **)&c->stats_dev, sizeof(unsigned long long) * STATS_N * STATS_SLOTS));
     if (nb == 9) TRY(mlp_alloc(c->mlp, [c](void **ptr, size_t bytes) { return dalloc(c, ptr, bytes); }));
-    if (nb == 9 && p.c2p_guess != AFD_C2P_PLAIN) {
+    if (nb == 9 && (p.c2p_guess != AFD_C2P_PLAIN || c->dmode)) {
         long blocks = (long)grid1d(c->g.ncell).x;
         c->c2p_qcap = (blocks + QUEUE_SLOTS - 1) / QUEUE_SLOTS * BLOCK;
         if (c->fused) {  // k_face_march_update (up to 64 x MARCH_CELLS cells a workgroup) + k_update_shell
@@ -318,8 +320,14 @@ extern "C" int afd_create(const afd_params *params, afd_ctx **out) {
             long shell = ((long)grid1d(shell_cells(c->g)).x + QUEUE_SLOTS - 1) / QUEUE_SLOTS * BLOCK;
             c->c2p_qcap = std::max(c->c2p_qcap, fused + shell);
         }
-        TRY(dalloc(c, (void **)&c->c2p_queue, sizeof(int) * (size_t)c->c2p_qcap * QUEUE_SLOTS));
-        TRY(dalloc(c, (void **)&c->c2p_qcount, sizeof(unsigned) * QUEUE_SLOTS));
+        if (p.c2p_guess != AFD_C2P_PLAIN) {
+            TRY(dalloc(c, (void **)&c->c2p_queue, sizeof(int) * (size_t)c->c2p_qcap * QUEUE_SLOTS));
+            TRY(dalloc(c, (void **)&c->c2p_qcount, sizeof(unsigned) * QUEUE_SLOTS));
+        }
+        if (c->dmode) {
+            TRY(dalloc(c, (void **)&c->repair_queue, sizeof(int) * (size_t)c->c2p_qcap * QUEUE_SLOTS));
+            TRY(dalloc(c, (void **)&c->repair_qcount, sizeof(unsigned) * QUEUE_SLOTS));
+        }
     }
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -679,6 +687,8 @@ static UpdLaunch upd_launch(afd_ctx *c, const double *Pin, const double *P0, dou
     a.guess = c->p.c2p_guess;
     a.queue = c->p.c2p_guess != AFD_C2P_PLAIN ? c->c2p_queue : nullptr;
     a.qcount = c->p.c2p_guess != AFD_C2P_PLAIN ? c->c2p_qcount : nullptr;
+    a.rq = c->repair_queue;
+    a.rqcount = c->repair_qcount;
     a.ng = c->p.ng;
     a.gr = c->p.gr_metric == 0 ? 0 : (c->p.gr_metric == 1 ? 1 : 2);
     a.gr_x0 = c->p.gr_x0;
@@ -712,6 +722,8 @@ extern "C" int afd_step(afd_ctx *c, double dt) {
         }
         Timed t(c, 3);
         hipError_t e;
+        if (c->repair_qcount)
+            HIPCHK(hipMemsetAsync(c->repair_qcount, 0, sizeof(unsigned) * QUEUE_SLOTS, c->stream));
         if (c->fused) {  // the y faces with the stage update
             FaceLaunch fl = face_launch(c, c->Puse ? c->Puse : c->P[src]);
             fl.dir = 1;
@@ -721,6 +733,8 @@ extern "C" int afd_step(afd_ctx *c, double dt) {
         } else {
             e = launch_update(l, false, c->stream);
         }
+        if (e == hipSuccess && c->repair_queue)  // cells whose FP32 right-hand side was not finite
+            e = launch_repair(face_launch(c, c->Puse ? c->Puse : c->P[src]), l, c->stream);
         if (e != hipSuccess) return fail(AFD_ERR_HIP, std::string("update kernel: ") + hipGetErrorString(e));
         src = dst;
     }

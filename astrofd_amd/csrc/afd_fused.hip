@@ -48,4 +48,22 @@ hipError_t launch_stage_fused(const FaceLaunch &l, const UpdLaunch &u, hipStream
     return hipGetLastError();
 }
 
+// The FP64 repair of the cells k_update (or the fused kernels) queued with a non-finite FP32
+// right-hand side; nothing runs past each slot's count, so with no such cell the launch only
+// reads the counters.
+hipError_t launch_repair(const FaceLaunch &l, const UpdLaunch &u, hipStream_t s) {
+    UpdArgs<float> ua;
+    static_assert(sizeof(UpdArgs<float>) == sizeof(UpdArgs<double>), "layout");
+    memcpy((void *)&ua, (const void *)&u.a, sizeof(ua));
+    if (!ua.rq) return hipSuccess;
+    FaceArgs<double> fa = face_args<double>(l);
+    const dim3 grid(16, QUEUE_SLOTS);
+    switch (l.riemann < 0 ? 0 : (l.riemann > 2 ? 2 : l.riemann)) {
+        case 0: hipLaunchKernelGGL((k_repair_weno<0>), grid, dim3(BLOCK), 0, s, fa, ua); break;
+        case 1: hipLaunchKernelGGL((k_repair_weno<1>), grid, dim3(BLOCK), 0, s, fa, ua); break;
+        default: hipLaunchKernelGGL((k_repair_weno<2>), grid, dim3(BLOCK), 0, s, fa, ua); break;
+    }
+    return hipGetLastError();
+}
+
 }  // namespace afd

@@ -777,6 +777,8 @@ template <typename F> struct UpdArgs {
     int *queue;                   // learned guess: cells whose start failed the acceptance test, QUEUE_SLOTS regions
     unsigned *qcount;             //   of qcap entries each, filled per workgroup slot (or null)
     long qcap;
+    int *rq;                      // mixed RMHD flux-divergence path: cells whose FP32 right-hand side is not
+    unsigned *rqcount;            //   finite, queued as `queue` is, for k_repair_weno (or null)
     int ng, gr, gr_x0;            // GR sources: 0 none, 1 isotropic-Schwarzschild lapse, 2 Kerr-Schild
     double gr_mass, gr_gfac;      // GR_MASS; Gamma / (Gamma - 1) of the sources' enthalpy
     unsigned long long *amax;     // SRHD last stage: max characteristic speed over sbox of the new
@@ -1065,6 +1067,20 @@ template <int NB, typename F, bool LEARNED, bool LDSW = false>
 AFD_D void cell_update(const UpdArgs<F> &a, long c, bool live, const double *rhs, MlpLds *lw = nullptr,
                        const MlpStaged<BLOCK> *ws = nullptr) {
     const Grid &g = a.g;
+    if constexpr (NB == 9 && sizeof(F) == 4) {
+        // a cell whose FP32 right-hand side overflowed (states beyond FP32's range, or a face
+        // that no FP32 fallback saved) is left untouched here and queued for k_repair_weno,
+        // which evaluates its faces in FP64 as the plain path does and then updates it
+        if (a.rq) {
+            bool bad = false;
+            if (live) {
+#pragma unroll
+                for (int v = 0; v < NB; ++v) bad = bad || !isfinite(rhs[v]);
+            }
+            wave_enqueue(bad, c, a.rq, a.rqcount, a.qcap);
+            live = live && !bad;
+        }
+    }
     C2PStats st = {0, 0, 0};
     double U[NB], q[NB];
 #pragma unroll
@@ -1287,6 +1303,85 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(LEARNED ?
     double rhs[NB];
     if (live) cell_rhs<NB, F>(a, c, false, rhs);
     cell_update<NB, F, LEARNED>(a, c, live, rhs);
+}
+
+// FP64 right-hand side of cell c on the mixed RMHD WENO5 path, as the plain path computes it
+// (k_face_rmhd<2, RIEM, dir, double> and cell_rhs_flat): the own-speed fluxes of its right faces,
+// the y and z fluxes of its left faces, and the x left face's flux with the speeds of its
+// predecessor's z-back face (Q3, rmhd_face_store), all from FP64 face states; then GLM damping,
+// resistivity and GR sources.
+template <int RIEM>
+AFD_D void repair_rhs(const FaceArgs<double> &fa, const UpdArgs<float> &ua, long c, double *rhs) {
+#pragma clang fp contract(off)
+    const Grid &g = fa.g;
+    int i, j, k;
+    cell_ijk(g, c, i, j, k);
+    double Fr[3][9], Fl[3][9];
+    {
+        RmhdFace<0, double> f;
+        rmhd_face_setup<2, 0, double>(fa, c + g.sx, f);
+        rmhd_riemann<2, RIEM, 0, double>(fa.ph, f, f.sL, f.sR, Fr[0]);
+    }
+    {
+        double cL = -1.0, cR = 1.0;  // hlle_bounds_fast without a predecessor
+        long pc = -1;
+        if (k > g.r0[2]) pc = c - 1;
+        else if (j > g.r0[1]) pc = cidx(g, i, j - 1, g.r1[2] - 1);
+        else if (i > g.r0[0]) pc = cidx(g, i - 1, g.r1[1] - 1, g.r1[2] - 1);
+        if (pc >= 0) {
+            RmhdFace<2, double> f;
+            rmhd_face_setup<2, 2, double>(fa, pc, f);
+            cL = f.sL;
+            cR = f.sR;
+        }
+        RmhdFace<0, double> f;
+        rmhd_face_setup<2, 0, double>(fa, c, f);
+        rmhd_riemann<2, RIEM, 0, double>(fa.ph, f, cL, cR, Fl[0]);
+    }
+    {
+        RmhdFace<1, double> f;
+        rmhd_face_setup<2, 1, double>(fa, c + g.sy, f);
+        rmhd_riemann<2, RIEM, 1, double>(fa.ph, f, f.sL, f.sR, Fr[1]);
+        rmhd_face_setup<2, 1, double>(fa, c, f);
+        rmhd_riemann<2, RIEM, 1, double>(fa.ph, f, f.sL, f.sR, Fl[1]);
+    }
+    {
+        RmhdFace<2, double> f;
+        rmhd_face_setup<2, 2, double>(fa, c + 1, f);
+        rmhd_riemann<2, RIEM, 2, double>(fa.ph, f, f.sL, f.sR, Fr[2]);
+        rmhd_face_setup<2, 2, double>(fa, c, f);
+        rmhd_riemann<2, RIEM, 2, double>(fa.ph, f, f.sL, f.sR, Fl[2]);
+    }
+#pragma unroll
+    for (int v = 0; v < 9; ++v) {
+        rhs[v] = 0.0;
+        rhs[v] -= (Fr[0][v] - Fl[0][v]) / ua.dx;
+        rhs[v] -= (Fr[1][v] - Fl[1][v]) / ua.dy;
+        rhs[v] -= (Fr[2][v] - Fl[2][v]) / ua.dz;
+    }
+    cell_rhs_glm_resist<9, float>(ua, c, rhs);
+    if (ua.gr) gr_source<9, float>(ua, c, rhs);
+}
+
+// The cells k_update queued in ua.rq: FP64 right-hand side (repair_rhs), then the stage update
+// (plain or previous-state start, con2prim inline).  blockIdx.y = queue slot; the block strides
+// over the slot's entries with every lane in each round (cell_update is wave-wide).
+template <int RIEM>
+__global__ __launch_bounds__(BLOCK) void k_repair_weno(FaceArgs<double> fa, UpdArgs<float> ua) {
+    const int slot = blockIdx.y;
+    const long cnt = ua.rqcount[slot];
+    UpdArgs<float> u = ua;
+    u.queue = nullptr;
+    u.rq = nullptr;
+    u.mlp = nullptr;
+    for (long base = (long)blockIdx.x * BLOCK; base < cnt; base += (long)gridDim.x * BLOCK) {
+        const long e = base + threadIdx.x;
+        const bool live = e < cnt;
+        const long c = live ? (long)ua.rq[(long)slot * ua.qcap + e] : 0;
+        double rhs[9];
+        if (live) repair_rhs<RIEM>(fa, ua, c, rhs);
+        cell_update<9, float, false>(u, c, live, rhs);
+    }
 }
 
 // Plain RMHD con2prim of the queued cells (rejected learned starts), densely packed:

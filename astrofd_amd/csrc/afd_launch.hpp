@@ -66,6 +66,9 @@ hipError_t launch_update(const UpdLaunch &l, bool rhs_only, hipStream_t s);
 // Mixed RMHD WENO5 on the flux-divergence path: the y faces fused with the stage update
 // (k_face_march_update), the ghost shell's update and the learned start's fallback queue.
 hipError_t launch_stage_fused(const FaceLaunch &l, const UpdLaunch &u, hipStream_t s);
+// ... and the FP64 repair of the cells whose FP32 right-hand side was not finite (k_repair_weno;
+// l: the stage's faces, its primitives)
+hipError_t launch_repair(const FaceLaunch &l, const UpdLaunch &u, hipStream_t s);
 long shell_cells(const Grid &g);
 long fused_blocks(const Grid &g);
 

@@ -114,6 +114,19 @@ def test_mixed_weno_large_jumps(phys, riem, face_path):
                      finite_only=riem == "hlld_full")
 
 
+@pytest.mark.parametrize("riem,face_path", [("hlle", 3), ("hlle", 4), ("hlld_full", 3)])
+def test_mixed_fp32_overflow_repaired_in_fp64(riem, face_path):
+    """A box of density 1e40, beyond FP32's range (3.4e38): every FP32 face whose stencil touches
+    it overflows, and the cells around it get a non-finite FP32 right-hand side.  On the
+    flux-divergence path (face_path 3, and 4 with the y faces in the update) the stage update
+    queues those cells and k_repair_weno evaluates their faces in FP64 as the plain path does, so
+    the mixed step is finite wherever the FP64 step is.  Bars as for the 1e10 jumps; with
+    hll_d_flux_dir, whose FP64 step blows up at the contact, finiteness only."""
+    log(f"test_mixed_fp32_overflow_repaired_in_fp64 {riem} face_path={face_path}")
+    check_mixed_step("rmhd", "weno", "mc", riem, 3, (22, 20, 18), vamp=0.2, face_path=face_path, jump=1e40,
+                     finite_only=riem == "hlld_full")
+
+
 @pytest.mark.parametrize("phys,riem", [("hydro", "hllc"), ("rmhd", "hlle")])
 def test_mixed_vanleer_large_jumps(phys, riem):
     """MUSCL with the van Leer limiter across density jumps of 1e20: the FP32 slopes' product
@@ -229,9 +242,13 @@ def check_mixed_step(phys, recon, lim, riem, rk, interior, vamp=0.5, npass=0, fa
                       limiter=lim, recon=recon, riemann=riem, rk_order=rk, n_passive=npass)
     ref = O.step(P, pr, dx, dx, dx, dt)
     core = (slice(ng, -ng),) * 3
-    assert np.all(np.isfinite(out))
+    # finite wherever the FP64 step is (everywhere, in every case here but the 1e40 box with
+    # hll_d_flux_dir, whose FP64 step may itself overflow there)
+    log(f"  non-finite: mixed {int(np.sum(~np.isfinite(out)))}, FP64 {int(np.sum(~np.isfinite(ref)))}")
+    assert np.all(np.isfinite(out) | ~np.isfinite(ref))
     if finite_only:
         return
+    assert np.all(np.isfinite(out))
     if jump is not None:
         # p and W of the cold dense gas come out of conserved states whose fluxes are 1e10 times
         # those around them: the bar is the FP32 resolution of the conserved update, every
