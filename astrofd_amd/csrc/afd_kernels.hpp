@@ -1160,10 +1160,10 @@ AFD_D void cell_update(const UpdArgs<F> &a, long c, bool live, const double *rhs
 // Stage update of one cell per thread: flux divergence + sources, then cell_update.  LEARNED
 // selects the network's Newton start; a separate instantiation keeps its registers out of the
 // plain kernel.  The learned instantiation is held to 4 waves per SIMD (<= 128 VGPRs; it takes
-// 100, no spills, with the network's weights read from LDS and its two tiles evaluated one after
-// the other): on the 256^3 RMHD WENO5 + HLLD step it ran 16 % slower at the compiler's own 2
-// waves, and 4 % slower at 3 waves with the tiles interleaved; 5 waves spills 20 bytes and is 3 %
-// slower.  The plain one is held to 5
+// 112, no spills, with the network's weights read from LDS, their loads held in registers until
+// the network runs, and its two tiles evaluated one after the other): on the 256^3 RMHD WENO5 +
+// HLLD step it ran 16 % slower at the compiler's own 2 waves, and 4 % slower at 3 waves with the
+// tiles interleaved; 5 waves spilled 20 bytes and was 3 % slower.  The plain one is held to 5
 // (<= 96 VGPRs, no spills; 100 registers and 4 waves otherwise): 6 % faster in FP32, 2.4 % in
 // FP64 on that step (6 waves spills and gains nothing more).
 #ifndef AFD_UPDATE_LEARNED_WAVES
