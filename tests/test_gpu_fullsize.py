@@ -1,0 +1,204 @@
+"""The benchmark workloads at their full 256^3 size on the GPU, checked against the oracle on
+sub-boxes of the same pre-step state.
+
+One SSPRK step of the reference changes a cell only through cells along each axis within the
+stages' stencil radius (3 stages x 3 cells for WENO5, x 2 for MUSCL).  The carried x speeds (Q3)
+come from the previous cell of the same z line, except at a line's first cell, which takes them
+from the last cell of the previous y line: with them a stage reaches one cell further in y (3 x 4
+cells for RMHD WENO5).  So the oracle's step on a box of the grid that spans all of z (the lines
+start where the grid's do) equals the full-size step on the box's cells at least that radius away
+from the ends of the box's own flux range on its cut sides in x and y (the box's step leaves the
+cells outside that range unchanged, where the grid's step updates them).  A box side that is the
+array's own end needs no margin: there the ranges and ghost layers are the grid's.  The two agree to rounding (a few cells differ by an ulp
+where the oracle's compiled loops round differently at another grid size).
+
+Boxes (x range, y range, all of z): the centre, the high-x / high-y corner, and the nozzle side
+(low x).  The GPU steps the whole 262^3 (RMHD) or 260^3 (SRHD) grid, ghost layers as the fills
+left them, and the oracle steps the same values.  Bars as in the reduced-grid config tests
+(tests/test_gpu_configs.py): FP64 max |difference| <= 1e-9 (1e-8 SRHD) of the variable's largest
+magnitude over the grid (the ambient's velocities are rounding noise, ~1e-17), mixed L1(rho) <=
+1e-6 and L1(p, W) <= 1e-5, and for hll_d_flux_dir as shipped 10x the oracle's own L1 response to
+1-ulp input noise on the same boxes.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from astrofd_amd import ic
+from astrofd_amd.domain import Domain
+from astrofd_amd.settings import load_settings
+from tests import oracle_loop
+from tests.gpu_helpers import log
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CFG = os.path.join(ROOT, "astrofd_amd", "config")
+BOX = 48
+RMHD_REACH = (3 * 3, 3 * 4)  # WENO5 x 3 stages; in y one more cell a stage for the carried speeds
+SRHD_REACH = (3 * 2, 3 * 2)  # MUSCL x 3 stages
+
+
+def settings(name, **override):
+    s = load_settings(["--config", os.path.join(CFG, name + ".json")])
+    s.update(override)
+    return s
+
+
+def boxes(n):
+    """(x0, x1, y0, y1) of the three boxes in an array of n cells per axis."""
+    mid = n // 2 - BOX // 2
+    return [(mid, mid + BOX, mid, mid + BOX), (n - BOX, n, n - BOX, n), (0, BOX, mid, mid + BOX)]
+
+
+def valid(lo, hi, n, reach, r0, r1_off, ng):
+    """Box coordinates [a, b) along one axis of the cells the box's step gets right, interior only:
+    reach cells inside the box's flux range [r0, box size - r1_off) on a cut side."""
+    a = lo if lo == 0 else lo + r0 + reach
+    b = hi if hi == n else hi - r1_off - reach
+    return max(a, ng) - lo, min(b, n - ng) - lo
+
+
+def gpu_step(s, pre, dt, precision, guess="plain"):
+    d = Domain.from_settings(dict(s, PRECISION=precision, C2P_GUESS=guess))
+    try:
+        if guess == "learned":
+            from astrofd_amd import learned
+            d.set_mlp_weights(learned.load(learned.MODEL_PATH))
+        d.set_prim(pre)
+        d.step(dt)
+        return d.get_prim()
+    finally:
+        d.close()
+
+
+def prestep(s):
+    """The seeded state after the driver's fills, and the step's dt (SRHD: from max_char_speed)."""
+    d = Domain.from_settings(dict(s, PRECISION="fp64"))
+    try:
+        d.set_prim(ic.initial_state(s))
+        d.set_nozzle(s)
+        d.apply_bcs()
+        amax = d.max_char_speed() if s["PHYSICS"] == "hydro" else 1.0
+        dx = min(s["Lx"] / s["NX"], s["Ly"] / s["NY"], s["Lz"] / s["NZ"])
+        return d.get_prim(), s["CFL"] * dx / max(amax, 1e-12), amax
+    finally:
+        d.close()
+
+
+def flux_range(s):
+    """Offsets of the flux range [r0, n - r1_off) from the array's ends (core/srhd_core.py:477-479,
+    core/rmhd_core.py:751-753; PPM/WENO [3, n - 3))."""
+    if s["RECON"] == "muscl":
+        return 2, (2 if s["PHYSICS"] == "rmhd" else 3)
+    return 3, 3
+
+
+def box_refs(s, pre, dt, reach, perturb=None, eps=2.220446049250313e-16):
+    """Per box: (box slices in the full grid, valid cells in box coordinates, oracle step).
+    reach: (x, y) cells of the step's reach; perturb: seed of relative input noise of +-eps."""
+    n, ng = pre.shape[1], s["NG"]
+    r0, r1_off = flux_range(s)
+    dx, dy, dz = s["Lx"] / s["NX"], s["Ly"] / s["NY"], s["Lz"] / s["NZ"]
+    out = []
+    for x0, x1, y0, y1 in boxes(n):
+        sub = np.ascontiguousarray(pre[:, x0:x1, y0:y1, :])
+        if perturb is not None:
+            sub = sub * (1.0 + eps * np.random.default_rng(perturb).choice([-1.0, 1.0], sub.shape))
+        P = oracle_loop.params_from_settings(s, sub.shape)
+        ref = O.step(P, sub, dx, dy, dz, dt)
+        va, vb = valid(x0, x1, n, reach[0], r0, r1_off, ng), valid(y0, y1, n, reach[1], r0, r1_off, ng)
+        cells = (slice(None), slice(*va), slice(*vb), slice(ng, n - ng))
+        out.append(((slice(None), slice(x0, x1), slice(y0, y1), slice(None)), cells, ref))
+    return out
+
+
+def lorentz(q):
+    v2 = q[1] ** 2 + q[2] ** 2 + q[3] ** 2
+    return 1.0 / np.sqrt(1.0 - np.minimum(v2, 1.0 - 1e-14))
+
+
+def l1(a, b):
+    return float(np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-300))
+
+
+def compare(got, refs):
+    """Per box: max over variables of max |got - ref| / (the variable's largest |got| over the grid),
+    L1 of rho, p, W, and L1 per variable."""
+    res = []
+    scale = [max(float(np.abs(got[v]).max()), 1e-300) for v in range(got.shape[0])]
+    for full, cells, ref in refs:
+        g, r = got[full][cells], ref[cells]
+        rel = max(float(np.abs(g[v] - r[v]).max()) / scale[v] for v in range(r.shape[0]))
+        res.append(dict(rel=rel, rho=l1(g[0], r[0]), p=l1(g[4], r[4]), W=l1(lorentz(g), lorentz(r)),
+                        var=np.array([l1(g[v], r[v]) for v in range(r.shape[0])]),
+                        finite=bool(np.all(np.isfinite(g)))))
+    return res
+
+
+def test_srhd_jet_256_full_size():
+    s = settings("srhd_jet_256")
+    pre, dt, amax = prestep(s)
+    a_ref = O.max_char_speed(oracle_loop.params_from_settings(s, pre.shape), pre)
+    log(f"[gpu] srhd_jet_256 {pre.shape}: max_char_speed {amax!r} oracle {a_ref!r}")
+    assert abs(amax - a_ref) <= 1e-15
+    refs = box_refs(s, pre, dt, SRHD_REACH)
+    for precision in ("fp64", "mixed"):
+        res = compare(gpu_step(s, pre, dt, precision), refs)
+        for b, r in enumerate(res):
+            log(f"      {precision} box {b}: max rel {r['rel']:.2e}  L1 rho {r['rho']:.2e} p {r['p']:.2e} W {r['W']:.2e}")
+            assert r["finite"]
+            if precision == "fp64":
+                assert r["rel"] <= 1e-8
+            else:
+                assert r["rho"] <= 1e-6 and r["p"] <= 1e-5 and r["W"] <= 1e-5
+
+
+def test_rmhd_jet_256_hlle_full_size():
+    s = settings("rmhd_jet_256", RIEMANN="hlle")
+    pre, dt, _ = prestep(s)
+    refs = box_refs(s, pre, dt, RMHD_REACH)
+    for precision, guess in (("fp64", "plain"), ("mixed", "learned")):
+        res = compare(gpu_step(s, pre, dt, precision, guess), refs)
+        for b, r in enumerate(res):
+            log(f"[gpu] rmhd_jet_256 hlle {precision}/{guess} box {b}: max rel {r['rel']:.2e}  "
+                f"L1 rho {r['rho']:.2e} p {r['p']:.2e} W {r['W']:.2e}")
+            assert r["finite"]
+            if precision == "fp64":
+                assert r["rel"] <= 1e-9
+            else:
+                assert r["rho"] <= 1e-6 and r["p"] <= 1e-5 and r["W"] <= 1e-5
+
+
+def test_rmhd_jet_256_as_shipped_full_size():
+    """hll_d_flux_dir, one step: the scheme is ill-conditioned over the static magnetised ambient
+    (tests/test_gpu_configs.py).  The FP64 path's L1 difference per variable stays within 10x the
+    oracle's own response to 1-ulp input noise on the same box.  The mixed path rounds at FP32
+    resolution, where the solver's branch thresholds switch more faces (next to the nozzle its p
+    moves 20x further than the 1-ulp response): its difference stays within 10x the oracle's
+    response to input noise of FP32 size (2^-24 relative), and L1(rho) within the metric's 1e-6."""
+    s = settings("rmhd_jet_256")
+    pre, dt, _ = prestep(s)
+    refs = box_refs(s, pre, dt, RMHD_REACH)
+    spreads = {}
+    for precision, eps in (("fp64", 2.220446049250313e-16), ("mixed", 2.0 ** -24)):
+        spread = [np.zeros(pre.shape[0]) for _ in refs]
+        for seed in (1, 2):
+            noisy = box_refs(s, pre, dt, RMHD_REACH, perturb=seed, eps=eps)
+            for b, ((_, cells, ref), (_, _, alt)) in enumerate(zip(refs, noisy)):
+                spread[b] = np.maximum(spread[b], [l1(alt[cells][v], ref[cells][v]) for v in range(ref.shape[0])])
+        spreads[precision] = spread
+    for precision, guess in (("fp64", "plain"), ("mixed", "learned")):
+        res = compare(gpu_step(s, pre, dt, precision, guess), refs)
+        spread = spreads[precision]
+        for b, r in enumerate(res):
+            ratio = r["var"] / np.maximum(spread[b], 1e-300)
+            log(f"[gpu] rmhd_jet_256 as shipped {precision}/{guess} box {b}: L1 rho {r['rho']:.2e} "
+                f"(spread {spread[b][0]:.2e}), ratio to spread max {float(ratio.max()):.2f}")
+            log("      L1 per variable " + " ".join(f"{x:.1e}" for x in r["var"]) + " | spread "
+                + " ".join(f"{x:.1e}" for x in spread[b]))
+            assert r["finite"]
+            assert np.all(r["var"] <= 10.0 * spread[b] + 1e-12)
+            if precision == "mixed":
+                assert r["rho"] <= 1e-6
