@@ -109,7 +109,7 @@ def box_refs(s, pre, dt, reach, perturb=None, eps=2.220446049250313e-16):
         P = oracle_loop.params_from_settings(s, sub.shape)
         ref = O.step(P, sub, dx, dy, dz, dt)
         va, vb = valid(x0, x1, n, reach[0], r0, r1_off, ng), valid(y0, y1, n, reach[1], r0, r1_off, ng)
-        cells = (slice(None), slice(*va), slice(*vb), slice(ng, n - ng))
+        cells = (slice(None), slice(*va), slice(*vb), slice(ng, pre.shape[3] - ng))
         out.append(((slice(None), slice(x0, x1), slice(y0, y1), slice(None)), cells, ref))
     return out
 
@@ -163,6 +163,24 @@ def test_rmhd_jet_256_hlle_full_size():
         res = compare(gpu_step(s, pre, dt, precision, guess), refs)
         for b, r in enumerate(res):
             log(f"[gpu] rmhd_jet_256 hlle {precision}/{guess} box {b}: max rel {r['rel']:.2e}  "
+                f"L1 rho {r['rho']:.2e} p {r['p']:.2e} W {r['W']:.2e}")
+            assert r["finite"]
+            if precision == "fp64":
+                assert r["rel"] <= 1e-9
+            else:
+                assert r["rho"] <= 1e-6 and r["p"] <= 1e-5 and r["W"] <= 1e-5
+
+
+def test_rmhd_jet_512_hlle_full_size():
+    """The largest workload, 518^3 cells with ghosts and 11 variables (resistivity once, Te/Ti
+    passives), with HLLE: the resistive Laplacian reaches one more cell a stage."""
+    s = settings("rmhd_jet_512", RIEMANN="hlle")
+    pre, dt, _ = prestep(s)
+    refs = box_refs(s, pre, dt, (3 * 4, 3 * 5))
+    for precision, guess in (("fp64", "plain"), ("mixed", "learned")):
+        res = compare(gpu_step(s, pre, dt, precision, guess), refs)
+        for b, r in enumerate(res):
+            log(f"[gpu] rmhd_jet_512 hlle {precision}/{guess} box {b}: max rel {r['rel']:.2e}  "
                 f"L1 rho {r['rho']:.2e} p {r['p']:.2e} W {r['W']:.2e}")
             assert r["finite"]
             if precision == "fp64":
