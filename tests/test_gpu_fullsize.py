@@ -155,6 +155,22 @@ def test_srhd_jet_256_full_size():
                 assert r["rho"] <= 1e-6 and r["p"] <= 1e-5 and r["W"] <= 1e-5
 
 
+def test_srhd_jet_256_full_grid_three_steps():
+    """srhd_jet_256 as configured, the whole 260^3 grid: three steps of the driver's loop (CFL
+    speed, fills, step, fills) on the GPU against the oracle's own loop (10 s on the host)."""
+    from tests.test_gpu_configs import gpu_run, oracle_run, per_var_rel
+    s = settings("srhd_jet_256")
+    pr = ic.initial_state(s)
+    got, gdt = gpu_run(s, pr, 3)
+    ref, rdt = oracle_run(s, pr, 3)
+    err = per_var_rel(got, ref)
+    log(f"[gpu] srhd_jet_256 full grid, 3 steps FP64: dt rel {np.abs(gdt - rdt).max() / rdt.max():.2e}, "
+        f"state rel {err:.2e}")
+    assert np.all(np.isfinite(got))
+    assert np.abs(gdt - rdt).max() <= 1e-12 * rdt.max()
+    assert err <= 1e-8
+
+
 def test_rmhd_jet_256_hlle_full_size():
     s = settings("rmhd_jet_256", RIEMANN="hlle")
     pre, dt, _ = prestep(s)
