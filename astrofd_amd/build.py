@@ -25,9 +25,12 @@ INCLUDE = os.path.join(ROOT, "include")
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 # Packed FP32 VALU ops are disabled (-packed-fp32-ops): gfx950 code is written for scalar FP32/FP64.
+# MFMAs take their accumulators in ordinary VGPRs (-amdgpu-mfma-vgpr-form): with the accumulation
+# registers the network's forward kernel copied every result out with v_accvgpr_read (146 a wave)
+# and ran at 3 waves per SIMD instead of 5.
 HIPFLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-Wall", "-Wno-unused-function",
             "-Wno-unused-variable", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-I", INCLUDE, "-I", CSRC]
+            "-mllvm", "-amdgpu-mfma-vgpr-form=1", "-I", INCLUDE, "-I", CSRC]
 
 UNITS = [
     ("afd_capi.hip", []),
