@@ -129,7 +129,7 @@ def evaluate(w, n=1 << 20, seed=99, device=0, return_states=False):
     changes, so the learned path can only gain there.  Iterations are the solver's own counts:
     hydro fixed-point iterations + Newton f(Z) evaluations (plain), Newton evaluations from the
     network's guess (+ the plain solve when the acceptance test fails)."""
-    q = sample_primitives(n, seed, slow_frac=0.0)  # the cfg3 distribution as specified
+    q = sample_primitives(n, seed, slow_frac=0.0)  # the specified distribution only (W uniform in [1, 50])
     with make_context(device) as plain, make_context(device, learned=True) as lrn:
         lrn.set_mlp_weights(w)
         _, _, newton_valid = plain.mlp_dataset_from_prims(q)
