@@ -570,6 +570,7 @@ AFD_D void rmhd_face_pair(const FaceArgs<float> &a, const RmhdFace<DIR, float> &
 #define AFD_LANES_DIV_WAVES AFD_FACE_WAVES_F32
 #endif
 
+
 template <int RIEM, bool ADD>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(AFD_LANES_DIV_WAVES)))
 void k_face_rmhd_weno_lanes_div(FaceArgs<float> a) {
@@ -627,6 +628,9 @@ constexpr int MARCH_CELLS = AFD_MARCH_CELLS;
 #ifndef AFD_MARCH_WAVES
 #define AFD_MARCH_WAVES AFD_FACE_WAVES_F32
 #endif
+#ifndef AFD_MARCH_PRIO
+#define AFD_MARCH_PRIO 1
+#endif
 
 template <int RIEM, int DIR, bool ADD>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AFD_MARCH_WAVES)))
@@ -666,7 +670,12 @@ void k_face_rmhd_weno_march(FaceArgs<float> a) {
         for (int v = 0; v < 9; ++v) dold[v] = (ADD && DIR == 0 && upd) ? a.D[v * n + cl] : 0.0f;
         State9 L, R;
         float lo[9], hi[9];
+        // the wave's priority is raised while it loads its stencil, so that a round's loads go
+        // out ahead of the other waves' arithmetic: y pencils 2.6 % faster, x 0.5 % (the z lanes,
+        // independent waves, ran 15 % slower with it and do not take it)
+        if (AFD_MARCH_PRIO) __builtin_amdgcn_s_setprio(1);
         weno5_cell_edges9(a.P, n, c, s, L, R, lo, hi);
+        if (AFD_MARCH_PRIO) __builtin_amdgcn_s_setprio(0);
 #pragma unroll
         for (int v = 0; v < 9; ++v) edge[buf][w][v][lane] = hi[v];
         __syncthreads();
