@@ -35,6 +35,20 @@ def test_mixed_step_vs_oracle(phys, recon, lim, riem, rk):
     check_mixed_step(phys, recon, lim, riem, rk, (22, 20, 18))
 
 
+@pytest.mark.parametrize("face_path", [2, 3, 4])
+@pytest.mark.parametrize("interior", [(22, 20, 18), (19, 16, 70)])
+def test_mixed_resistive_laplacian(interior, face_path):
+    """The resistive Laplacian of B applied once per right-hand side (RESISTIVE_MODE "once") on
+    the FP32 WENO5 face variants: face fluxes stored (2), the flux divergence accumulated (3) and
+    the y faces in the update (4); the stage update adds the Laplacian in FP64 after the FP32
+    divergence.  Resistivity 0.02 on unequal spacings: the term moves B by 0.2 to 4 % (L1) in the
+    step, so a part dropped, counted twice, given the wrong sign or another axis's spacing fails
+    the bars above (1e-5) by orders of magnitude."""
+    log(f"test_mixed_resistive_laplacian {interior} face_path={face_path}")
+    check_mixed_step("rmhd", "weno", "mc", "hlld_full", 3, interior, vamp=0.2, face_path=face_path,
+                     resistivity=0.02, spacing=(1.0, 1.3, 0.8))
+
+
 @pytest.mark.parametrize("recon,lim,riem", [("weno", "mc", "hlld_full"), ("ppm", "mc", "hlld"), ("muscl", "vanleer", "hlld")])
 @pytest.mark.parametrize("interior", [(19, 16, 70), (21, 18, 130), (2, 3, 2)])
 def test_mixed_shared_edges_shapes(interior, recon, lim, riem):
@@ -223,7 +237,7 @@ def test_mixed_hllc_vacuum_faces(phys, riem):
 
 
 def check_mixed_step(phys, recon, lim, riem, rk, interior, vamp=0.5, npass=0, face_path=0, jump=None,
-                     finite_only=False):
+                     finite_only=False, resistivity=0.0, spacing=(1.0, 1.0, 1.0)):
     ng = 3
     shape = tuple(n + 2 * ng for n in interior)
     pr = smooth_state(shape, 91, phys, vamp=vamp, npass=npass)
@@ -231,16 +245,19 @@ def check_mixed_step(phys, recon, lim, riem, rk, interior, vamp=0.5, npass=0, fa
         pr[0][ng + 6:ng + 14, ng + 5:ng + 13, ng + 4:ng + 12] *= jump
     gamma = 4.0 / 3.0 if phys == "rmhd" else None
     dx = 1.0 / 22
-    dt = 0.4 * dx * 0.5
-    d = make_domain(phys, pr.shape, ng, dxyz=(dx, dx, dx), gamma=gamma, recon=recon, limiter=lim, riemann=riem,
-                    rk_order=rk, precision="mixed", n_passive=npass, face_path=face_path)
+    dxyz = tuple(dx * f for f in spacing)
+    dt = 0.4 * min(dxyz) * 0.5
+    d = make_domain(phys, pr.shape, ng, dxyz=dxyz, gamma=gamma, recon=recon, limiter=lim, riemann=riem,
+                    rk_order=rk, precision="mixed", n_passive=npass, face_path=face_path,
+                    n_resist=1 if resistivity else 0, resistivity=resistivity)
     d.set_prim(pr)
     d.step(dt)
     out = d.get_prim()
     d.close()
     P = O.make_params(phys, pr.shape[1:], nv=pr.shape[0], ng=ng, gamma=gamma, p_max=2.0, v_max=0.9999,
-                      limiter=lim, recon=recon, riemann=riem, rk_order=rk, n_passive=npass)
-    ref = O.step(P, pr, dx, dx, dx, dt)
+                      limiter=lim, recon=recon, riemann=riem, rk_order=rk, n_passive=npass,
+                      resistive=resistivity > 0.0, resistivity=resistivity, resist_once=resistivity > 0.0)
+    ref = O.step(P, pr, *dxyz, dt)
     core = (slice(ng, -ng),) * 3
     # finite wherever the FP64 step is (everywhere, in every case here but the 1e40 box with
     # hll_d_flux_dir, whose FP64 step may itself overflow there)
