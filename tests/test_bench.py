@@ -98,10 +98,19 @@ def test_roofline_picks_the_slowest_kernel_and_counts_its_bytes():
     assert r["bytes_per_launch"] == (9 * 8 + 2 * 9 * 8 + 2 * 8) * 257 * 256 * 256
 
 
-def test_traffic_names_its_source():
-    s = bench.load_workload("rmhd_jet_256")
+@pytest.mark.parametrize("workload", ["rmhd_jet_256", "rmhd_jet_512", "srhd_jet_256"])
+def test_traffic_names_its_source(workload):
+    """Every workload the bench line reports has a committed traffic profile holding the stage
+    update of both runs (mixed with its default start, FP64 plain) under the names the roofline
+    looks up, and the roofline names the file it read."""
+    s = bench.load_workload(workload)
+    rm = s["PHYSICS"] == "rmhd"
     kern = {"launches_face": [3, 3, 3], "ms_face": [3.0, 2.4, 2.1], "launches_update": 3, "ms_update": 6.3}
-    r = bench.kernel_roofline(s, kern, True, True, "rmhd_jet_256")
-    assert (r["traffic"] is None) == (r["traffic_source"] is None)
-    if r["traffic"] is not None:
-        assert r["traffic_source"].startswith("profiles/rmhd_jet_256_traffic.txt")
+    table, _ = bench.measured_traffic(workload)
+    for mixed in (True, False):
+        learned = mixed and rm
+        r = bench.kernel_roofline(s, kern, mixed, learned, workload)
+        assert r["kernel"] == "stage_update_c2p"
+        assert r["rocprof_name"] in table
+        assert r["traffic"] == round(table[r["rocprof_name"]]) and r["traffic"] > 0
+        assert r["traffic_source"].startswith(f"profiles/{workload}_traffic.txt")
